@@ -1,0 +1,432 @@
+// ldg.hip -- C-ABI implementation of libldg.so (see include/ldg.h).
+//
+// Owns the device state of one solver instance and drives the two-kernel
+// RK4 stage pipeline on one HIP stream:
+//   u      u^n (the reference's Solver::u)
+//   acc    u^n + dt sum_s b_s k_s, becomes u^{n+1} after stage 4
+//   ua/ub  stage inputs U_s (double-buffered: stage s+1 is written while
+//          neighbours of stage s are still being read)
+//   q      LDG auxiliary gradient (15 fields per element)
+#include "ldg.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "basis.hpp"
+#include "ldg_kernels.hpp"
+
+using namespace ldg;
+
+namespace {
+
+std::string g_create_error;
+
+// elements per workgroup for each order (LDS-limited, see kernel headers)
+template <int N> struct Tile;
+template <> struct Tile<2> { static constexpr int E = 16, EG = 16; };
+template <> struct Tile<3> { static constexpr int E = 8, EG = 8; };
+template <> struct Tile<4> { static constexpr int E = 4, EG = 4; };
+template <> struct Tile<5> { static constexpr int E = 2, EG = 2; };
+template <> struct Tile<6> { static constexpr int E = 1, EG = 1; };
+
+}  // namespace
+
+struct ldg_handle {
+  ldg_config cfg{};
+  int n = 0, dofs = 0;
+  long K = 0;
+  size_t nU = 0;
+  double dx[3]{};
+  Basis1D basis;
+  hipStream_t stream = nullptr;
+  double *u = nullptr, *acc = nullptr, *ua = nullptr, *ub = nullptr, *q = nullptr, *k = nullptr;
+  double *ke_part = nullptr, *diag_part = nullptr, *ke_hist = nullptr, *scal = nullptr;
+  int nblocks_rhs = 0, nblocks_grad = 0;
+  size_t ke_hist_cap = 0;
+  std::string err;
+};
+
+namespace {
+
+int fail(ldg_handle *h, int code, const std::string &msg) {
+  if (h) h->err = msg; else g_create_error = msg;
+  return code;
+}
+
+#define HIPCHECK(h, call)                                                               \
+  do {                                                                                  \
+    hipError_t e_ = (call);                                                             \
+    if (e_ != hipSuccess)                                                               \
+      return fail((h), LDG_EDEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+template <int N>
+Ops<N, double> make_ops(const Basis1D &B) {
+  Ops<N, double> o;
+  for (int a = 0; a < N; ++a) {
+    for (int b = 0; b < N; ++b) {
+      o.I[a][b] = B.at(B.I, a, b);
+      o.D[a][b] = B.at(B.D, a, b);
+      o.Dn[a][b] = B.at(B.Dn, a, b);
+      o.Minv[a][b] = B.at(B.M1inv, a, b);
+    }
+    o.L0[a] = B.at(B.M1inv, a, 0);
+    o.w[a] = B.w[a];
+  }
+  return o;
+}
+
+MeshArgs mesh_args(const ldg_handle *h, int E) {
+  MeshArgs m;
+  m.nx = h->cfg.nx; m.ny = h->cfg.ny; m.nz = h->cfg.nz;
+  m.nelem = h->K;
+  m.nblocks = (int)((h->K + E - 1) / E);
+  m.per_xcd = (m.nblocks + 7) / 8;
+  for (int l = 0; l < 3; ++l) m.ialpha[l] = 2.0 / h->dx[l];
+  return m;
+}
+
+PhysArgs phys_args(const ldg_handle *h) {
+  PhysArgs p;
+  p.gamma = h->cfg.gamma;
+  p.gm1 = h->cfg.gamma - 1.0;
+  p.mu = 1.0 / h->cfg.Re;
+  p.kappa = p.mu * h->cfg.gamma / h->cfg.Pr;
+  const double c0 = h->cfg.V0 / h->cfg.M0;
+  p.pref = h->cfg.rho0 * c0 * c0 / h->cfg.gamma;   // solver.cpp:40-41
+  return p;
+}
+
+template <int N>
+int launch_grad_t(ldg_handle *h, const double *U) {
+  constexpr int E = Tile<N>::EG;
+  MeshArgs m = mesh_args(h, E);
+  dim3 grid(8 * m.per_xcd), block(NT);
+  hipLaunchKernelGGL((ldg_grad_kernel<N, E>), grid, block, 0, h->stream, U, h->q, m, make_ops<N>(h->basis));
+  HIPCHECK(h, hipGetLastError());
+  return LDG_OK;
+}
+
+template <int N>
+int launch_rhs_t(ldg_handle *h, const double *U, const StageArgs &st) {
+  constexpr int E = Tile<N>::E;
+  MeshArgs m = mesh_args(h, E);
+  dim3 grid(8 * m.per_xcd), block(NT);
+  hipLaunchKernelGGL((ldg_rhs_kernel<N, E>), grid, block, 0, h->stream, U, (const double *)h->q, m,
+                     make_ops<N>(h->basis), phys_args(h), st);
+  HIPCHECK(h, hipGetLastError());
+  return LDG_OK;
+}
+
+int launch_grad(ldg_handle *h, const double *U) {
+  switch (h->n) {
+    case 2: return launch_grad_t<2>(h, U);
+    case 3: return launch_grad_t<3>(h, U);
+    case 4: return launch_grad_t<4>(h, U);
+    case 5: return launch_grad_t<5>(h, U);
+    case 6: return launch_grad_t<6>(h, U);
+  }
+  return fail(h, LDG_EINVAL, "unsupported order");
+}
+
+int launch_rhs(ldg_handle *h, const double *U, const StageArgs &st) {
+  switch (h->n) {
+    case 2: return launch_rhs_t<2>(h, U, st);
+    case 3: return launch_rhs_t<3>(h, U, st);
+    case 4: return launch_rhs_t<4>(h, U, st);
+    case 5: return launch_rhs_t<5>(h, U, st);
+    case 6: return launch_rhs_t<6>(h, U, st);
+  }
+  return fail(h, LDG_EINVAL, "unsupported order");
+}
+
+int rhs_blocks(int n, long K) {
+  int E = n == 2 ? Tile<2>::E : n == 3 ? Tile<3>::E : n == 4 ? Tile<4>::E : n == 5 ? Tile<5>::E : Tile<6>::E;
+  return (int)((K + E - 1) / E);
+}
+
+double ke_scale(const ldg_handle *h) {
+  const double J = (h->dx[0] / 2) * (h->dx[1] / 2) * (h->dx[2] / 2);
+  return J / (h->cfg.rho0 * std::pow(2 * M_PI * h->cfg.L, 3.0));
+}
+
+StageArgs empty_stage() {
+  StageArgs s;
+  std::memset(&s, 0, sizeof(s));
+  return s;
+}
+
+// One RK4 step of the resident state; KE of u^{n+1} to ke_out (device) if set.
+int rk4_step(ldg_handle *h, double dt, double *ke_out) {
+  static const double b[4] = {1 / 6.0, 1 / 3.0, 1 / 3.0, 1 / 6.0};
+  static const double a[3] = {0.5, 0.5, 1.0};
+  const double *U = h->u;
+  double *bufs[2] = {h->ua, h->ub};
+  for (int s = 0; s < 4; ++s) {
+    int rc = launch_grad(h, U);
+    if (rc) return rc;
+    StageArgs st = empty_stage();
+    st.u0 = h->u;
+    st.acc = h->acc;
+    st.acc_init = (s == 0);
+    st.dtb = dt * b[s];
+    if (s < 3) { st.unext = bufs[s & 1]; st.dta = dt * a[s]; }
+    if (s == 3 && ke_out) st.ke_part = h->ke_part;
+    rc = launch_rhs(h, U, st);
+    if (rc) return rc;
+    U = bufs[s & 1];
+  }
+  if (ke_out) {
+    hipLaunchKernelGGL(ldg_finalize_kernel, dim3(1), dim3(NT), 0, h->stream, (const double *)h->ke_part,
+                       h->nblocks_rhs, 1, ke_scale(h), ke_out);
+    HIPCHECK(h, hipGetLastError());
+  }
+  std::swap(h->u, h->acc);
+  return LDG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ldg_config_default(ldg_config *c) {
+  if (!c) return LDG_EINVAL;
+  std::memset(c, 0, sizeof(*c));
+  c->p = 2;
+  c->nx = c->ny = c->nz = 10;
+  for (int l = 0; l < 3; ++l) { c->xmin[l] = -M_PI; c->xmax[l] = M_PI; }
+  c->L = 1.0;
+  c->Re = 1600.0; c->Pr = 0.71; c->gamma = 1.4; c->M0 = 0.1; c->rho0 = 1.0; c->V0 = 1.0; c->R = 8.314;
+  c->precision = LDG_FP64;
+  return LDG_OK;
+}
+
+const char *ldg_last_error(const ldg_handle *h) { return h ? h->err.c_str() : g_create_error.c_str(); }
+
+int ldg_create(const ldg_config *cfg, ldg_handle **out) {
+  if (!cfg || !out) return fail(nullptr, LDG_EINVAL, "null argument");
+  *out = nullptr;
+  if (cfg->p < 1 || cfg->p > 5) return fail(nullptr, LDG_EINVAL, "order p must be in 1..5");
+  if (cfg->nx < 3 || cfg->ny < 3 || cfg->nz < 3)
+    return fail(nullptr, LDG_EINVAL, "need at least 3 elements per direction (mesh.cpp:102-107)");
+  for (int l = 0; l < 3; ++l)
+    if (!(cfg->xmax[l] > cfg->xmin[l])) return fail(nullptr, LDG_EINVAL, "empty domain");
+  if (!(cfg->Re > 0) || !(cfg->Pr > 0) || !(cfg->gamma > 1) || !(cfg->M0 > 0) || !(cfg->rho0 > 0) ||
+      !(cfg->V0 > 0) || !(cfg->L > 0) || !(cfg->R > 0))
+    return fail(nullptr, LDG_EINVAL, "physics parameters must be positive (gamma > 1)");
+  if (cfg->precision != LDG_FP64) return fail(nullptr, LDG_EINVAL, "precision mode not available in this build");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(nullptr, LDG_EDEVICE, "no HIP device");
+
+  ldg_handle *h = new ldg_handle();
+  h->cfg = *cfg;
+  h->n = cfg->p + 1;
+  h->dofs = h->n * h->n * h->n;
+  h->K = (long)cfg->nx * cfg->ny * cfg->nz;
+  h->nU = (size_t)h->dofs * NSTATE * h->K;
+  const int ns[3] = {cfg->nx, cfg->ny, cfg->nz};
+  for (int l = 0; l < 3; ++l) h->dx[l] = (cfg->xmax[l] - cfg->xmin[l]) / ns[l];
+  h->basis = make_basis(cfg->p);
+  h->nblocks_rhs = rhs_blocks(h->n, h->K);
+  auto bad = [&](int code) { int c = code; g_create_error = h->err; ldg_destroy(h); return c; };
+  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+    h->err = "hipStreamCreate failed";
+    return bad(LDG_EDEVICE);
+  }
+  const size_t bytes = h->nU * sizeof(double);
+  double **arrs[5] = {&h->u, &h->acc, &h->ua, &h->ub, &h->k};
+  for (double **p : arrs)
+    if (hipMalloc((void **)p, bytes) != hipSuccess) { h->err = "device allocation failed (state)"; return bad(LDG_ENOMEM); }
+  if (hipMalloc((void **)&h->q, bytes * 3) != hipSuccess) { h->err = "device allocation failed (gradient)"; return bad(LDG_ENOMEM); }
+  if (hipMalloc((void **)&h->ke_part, sizeof(double) * (h->nblocks_rhs + 1)) != hipSuccess ||
+      hipMalloc((void **)&h->diag_part, sizeof(double) * 2 * (h->nblocks_rhs + 1)) != hipSuccess ||
+      hipMalloc((void **)&h->scal, sizeof(double) * 8) != hipSuccess) {
+    h->err = "device allocation failed (partials)";
+    return bad(LDG_ENOMEM);
+  }
+  if (hipMemsetAsync(h->u, 0, bytes, h->stream) != hipSuccess || hipStreamSynchronize(h->stream) != hipSuccess) {
+    h->err = "device initialisation failed";
+    return bad(LDG_EDEVICE);
+  }
+  *out = h;
+  return LDG_OK;
+}
+
+void ldg_destroy(ldg_handle *h) {
+  if (!h) return;
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  double *ptrs[] = {h->u, h->acc, h->ua, h->ub, h->q, h->k, h->ke_part, h->diag_part, h->ke_hist, h->scal};
+  for (double *p : ptrs) if (p) (void)hipFree(p);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+}
+
+int ldg_sizes(const ldg_handle *h, int *dofs, int *nelem) {
+  if (!h) return LDG_EINVAL;
+  if (dofs) *dofs = h->dofs;
+  if (nelem) *nelem = (int)h->K;
+  return LDG_OK;
+}
+
+int ldg_initial_condition(ldg_handle *h) {
+  if (!h) return LDG_EINVAL;
+  TGVArgs t;
+  std::memset(&t, 0, sizeof(t));
+  t.n = h->n; t.nx = h->cfg.nx; t.ny = h->cfg.ny; t.nz = h->cfg.nz; t.nelem = h->K;
+  for (int l = 0; l < 3; ++l) { t.xmin[l] = h->cfg.xmin[l]; t.xmax[l] = h->cfg.xmax[l]; }
+  for (int i = 0; i < h->n; ++i) t.r[i] = h->basis.r[i];
+  const double c0 = h->cfg.V0 / h->cfg.M0;
+  const double p0 = h->cfg.rho0 * c0 * c0 / h->cfg.gamma;     // solver.cpp:40-43
+  const double T0 = p0 / (h->cfg.R * h->cfg.rho0);
+  t.L = h->cfg.L; t.p0 = p0; t.rho0 = h->cfg.rho0; t.V0 = h->cfg.V0; t.RT0 = h->cfg.R * T0; t.gamma = h->cfg.gamma;
+  hipLaunchKernelGGL(ldg_tgv_kernel, dim3(2048), dim3(256), 0, h->stream, h->u, t);
+  HIPCHECK(h, hipGetLastError());
+  HIPCHECK(h, hipStreamSynchronize(h->stream));
+  return LDG_OK;
+}
+
+int ldg_set_state(ldg_handle *h, const double *u) {
+  if (!h || !u) return LDG_EINVAL;
+  HIPCHECK(h, hipMemcpyAsync(h->u, u, h->nU * sizeof(double), hipMemcpyHostToDevice, h->stream));
+  HIPCHECK(h, hipStreamSynchronize(h->stream));
+  return LDG_OK;
+}
+
+int ldg_get_state(ldg_handle *h, double *u) {
+  if (!h || !u) return LDG_EINVAL;
+  HIPCHECK(h, hipMemcpyAsync(u, h->u, h->nU * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  HIPCHECK(h, hipStreamSynchronize(h->stream));
+  return LDG_OK;
+}
+
+int ldg_state_device_ptr(ldg_handle *h, double **dptr) {
+  if (!h || !dptr) return LDG_EINVAL;
+  *dptr = h->u;
+  return LDG_OK;
+}
+
+double ldg_time_step(const ldg_handle *h, double cfl) {
+  if (!h) return -1.0;
+  const double m = std::fmin(std::fmin(h->dx[0], h->dx[1]), h->dx[2]);
+  return cfl * m / (h->cfg.V0 * (2 * h->cfg.p + 1));
+}
+
+int ldg_rhs(ldg_handle *h, const double *u, double *k, double *Du) {
+  if (!h || !u || !k) return LDG_EINVAL;
+  const size_t bytes = h->nU * sizeof(double);
+  HIPCHECK(h, hipMemcpyAsync(h->ua, u, bytes, hipMemcpyHostToDevice, h->stream));
+  int rc = launch_grad(h, h->ua);
+  if (rc) return rc;
+  StageArgs st = empty_stage();
+  st.kout = h->k;
+  rc = launch_rhs(h, h->ua, st);
+  if (rc) return rc;
+  HIPCHECK(h, hipMemcpyAsync(k, h->k, bytes, hipMemcpyDeviceToHost, h->stream));
+  if (Du) {
+    std::vector<double> qh(h->nU * 3);
+    HIPCHECK(h, hipMemcpyAsync(qh.data(), h->q, bytes * 3, hipMemcpyDeviceToHost, h->stream));
+    HIPCHECK(h, hipStreamSynchronize(h->stream));
+    // (node, s + 5 l, element) -> Dus(node, s, element, l)
+    const size_t dofs = h->dofs;
+    for (long e = 0; e < h->K; ++e)
+      for (int l = 0; l < 3; ++l)
+        for (int s = 0; s < NSTATE; ++s)
+          std::memcpy(Du + dofs * (s + NSTATE * (e + (size_t)h->K * l)),
+                      qh.data() + dofs * ((size_t)e * NGRAD + 5 * l + s), dofs * sizeof(double));
+  }
+  HIPCHECK(h, hipStreamSynchronize(h->stream));
+  return LDG_OK;
+}
+
+int ldg_step(ldg_handle *h, double dt, int nsteps, double *ke) {
+  if (!h || nsteps < 0 || !(dt > 0)) return fail(h, LDG_EINVAL, "bad step arguments");
+  if (ke && (size_t)nsteps > h->ke_hist_cap) {
+    if (h->ke_hist) (void)hipFree(h->ke_hist);
+    h->ke_hist = nullptr;
+    HIPCHECK(h, hipMalloc((void **)&h->ke_hist, sizeof(double) * nsteps));
+    h->ke_hist_cap = nsteps;
+  }
+  for (int it = 0; it < nsteps; ++it) {
+    int rc = rk4_step(h, dt, ke ? h->ke_hist + it : nullptr);
+    if (rc) return rc;
+  }
+  if (ke && nsteps > 0) {
+    HIPCHECK(h, hipMemcpyAsync(ke, h->ke_hist, sizeof(double) * nsteps, hipMemcpyDeviceToHost, h->stream));
+  }
+  HIPCHECK(h, hipStreamSynchronize(h->stream));
+  if (ke)
+    for (int i = 0; i < nsteps; ++i)
+      if (!std::isfinite(ke[i])) return fail(h, LDG_ENONFINITE, "kinetic energy became non-finite");
+  return LDG_OK;
+}
+
+int ldg_diagnostics(ldg_handle *h, double out[3]) {
+  if (!h || !out) return LDG_EINVAL;
+  // reference KE of u: a zero-length RK epilogue (acc = u + 0 k) gives the nodal sum
+  int rc = launch_grad(h, h->u);
+  if (rc) return rc;
+  StageArgs st = empty_stage();
+  st.u0 = h->u;
+  st.acc = h->ua;
+  st.acc_init = 1;
+  st.dtb = 0.0;
+  st.ke_part = h->ke_part;
+  st.diag_part = h->diag_part;
+  rc = launch_rhs(h, h->u, st);
+  if (rc) return rc;
+  const double sc = ke_scale(h);
+  hipLaunchKernelGGL(ldg_finalize_kernel, dim3(1), dim3(NT), 0, h->stream, (const double *)h->ke_part,
+                     h->nblocks_rhs, 1, sc, h->scal);
+  hipLaunchKernelGGL(ldg_finalize_kernel, dim3(1), dim3(NT), 0, h->stream, (const double *)h->diag_part,
+                     h->nblocks_rhs, 2, sc, h->scal + 1);
+  HIPCHECK(h, hipGetLastError());
+  HIPCHECK(h, hipMemcpyAsync(out, h->scal, 3 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  HIPCHECK(h, hipStreamSynchronize(h->stream));
+  for (int i = 0; i < 3; ++i)
+    if (!std::isfinite(out[i])) return fail(h, LDG_ENONFINITE, "diagnostic is non-finite");
+  return LDG_OK;
+}
+
+int ldg_kernel_times(ldg_handle *h, double dt, int nsteps, double ms[2]) {
+  if (!h || !ms || nsteps < 1) return LDG_EINVAL;
+  static const double b[4] = {1 / 6.0, 1 / 3.0, 1 / 3.0, 1 / 6.0};
+  static const double a[3] = {0.5, 0.5, 1.0};
+  hipEvent_t ev[3];
+  for (auto &e : ev) HIPCHECK(h, hipEventCreate(&e));
+  double tot[2] = {0, 0};
+  int count = 0;
+  for (int it = 0; it < nsteps; ++it) {
+    const double *U = h->u;
+    double *bufs[2] = {h->ua, h->ub};
+    for (int s = 0; s < 4; ++s) {
+      HIPCHECK(h, hipEventRecord(ev[0], h->stream));
+      int rc = launch_grad(h, U);
+      if (rc) return rc;
+      HIPCHECK(h, hipEventRecord(ev[1], h->stream));
+      StageArgs st = empty_stage();
+      st.u0 = h->u; st.acc = h->acc; st.acc_init = (s == 0); st.dtb = dt * b[s];
+      if (s < 3) { st.unext = bufs[s & 1]; st.dta = dt * a[s]; }
+      rc = launch_rhs(h, U, st);
+      if (rc) return rc;
+      HIPCHECK(h, hipEventRecord(ev[2], h->stream));
+      HIPCHECK(h, hipEventSynchronize(ev[2]));
+      float t0 = 0, t1 = 0;
+      HIPCHECK(h, hipEventElapsedTime(&t0, ev[0], ev[1]));
+      HIPCHECK(h, hipEventElapsedTime(&t1, ev[1], ev[2]));
+      tot[0] += t0; tot[1] += t1; ++count;
+      U = bufs[s & 1];
+    }
+    std::swap(h->u, h->acc);
+  }
+  for (auto &e : ev) (void)hipEventDestroy(e);
+  ms[0] = tot[0] / count;
+  ms[1] = tot[1] / count;
+  return LDG_OK;
+}
+
+}  // extern "C"

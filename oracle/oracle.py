@@ -11,8 +11,9 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "_build", "libldg_oracle.so")
-_lib = None
+_LIB_PATHS = {"plain": os.path.join(_HERE, "_build", "libldg_oracle.so"),
+              "fma": os.path.join(_HERE, "_build", "libldg_oracle_fma.so")}
+_libs = {}
 
 _dp = ctypes.POINTER(ctypes.c_double)
 
@@ -21,12 +22,14 @@ def build():
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(_LIB_PATH):
+def lib(variant="plain"):
+    """The oracle library; variant "fma" is the same source built with fused
+    multiply-adds (used to measure the oracle's own round-off spread)."""
+    if variant not in _libs:
+        path = _LIB_PATHS[variant]
+        if not os.path.exists(path):
             build()
-        L = ctypes.CDLL(_LIB_PATH)
+        L = ctypes.CDLL(path)
         L.orc_create.restype = ctypes.c_void_p
         L.orc_create.argtypes = [ctypes.c_int] * 4 + [_dp, _dp, ctypes.c_double, _dp]
         L.orc_destroy.argtypes = [ctypes.c_void_p]
@@ -50,8 +53,8 @@ def lib():
         L.orc_fluxC.argtypes = [_dp, ctypes.c_double, _dp]
         L.orc_fluxV.argtypes = [_dp, _dp, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp]
         L.orc_roe.argtypes = [_dp, _dp, _dp, ctypes.c_double, _dp]
-        _lib = L
-    return _lib
+        _libs[variant] = L
+    return _libs[variant]
 
 
 def _p(a):
@@ -89,7 +92,7 @@ class Oracle:
     u(iN, iS, iK) column-major as in array.h.
     """
 
-    def __init__(self, p, nx, ny=None, nz=None, xmin=None, xmax=None, L=1.0, phys=None):
+    def __init__(self, p, nx, ny=None, nz=None, xmin=None, xmax=None, L=1.0, phys=None, variant="plain"):
         ny = nx if ny is None else ny
         nz = nx if nz is None else nz
         xmin = np.full(3, -np.pi * L) if xmin is None else np.asarray(xmin, dtype=np.float64)
@@ -99,17 +102,18 @@ class Oracle:
         ph = None
         if phys is not None:
             ph = np.ascontiguousarray(phys, dtype=np.float64)
-        self._h = lib().orc_create(p, nx, ny, nz, _p(xmin), _p(xmax), L, _p(ph) if ph is not None else None)
+        self._lib = lib(variant)
+        self._h = self._lib.orc_create(p, nx, ny, nz, _p(xmin), _p(xmax), L, _p(ph) if ph is not None else None)
         if not self._h:
             raise ValueError("oracle setup failed")
         self.p, self.n = p, p + 1
         self.nx, self.ny, self.nz = nx, ny, nz
-        self.dofs = lib().orc_dofs(self._h)
-        self.K = lib().orc_nelem(self._h)
+        self.dofs = self._lib.orc_dofs(self._h)
+        self.K = self._lib.orc_nelem(self._h)
 
     def __del__(self):
         if getattr(self, "_h", None):
-            lib().orc_destroy(self._h)
+            self._lib.orc_destroy(self._h)
             self._h = None
 
     @property
@@ -118,46 +122,46 @@ class Oracle:
 
     def initial_condition(self):
         u = np.zeros(self.shape)
-        lib().orc_initial_condition(self._h, _p(u))
+        self._lib.orc_initial_condition(self._h, _p(u))
         return u
 
     def rhs(self, u, want_du=False):
         u = np.ascontiguousarray(u, dtype=np.float64)
         k = np.zeros(self.shape)
         du = np.zeros((3,) + self.shape) if want_du else None
-        lib().orc_rhs(self._h, _p(u), _p(k), _p(du) if want_du else None)
+        self._lib.orc_rhs(self._h, _p(u), _p(k), _p(du) if want_du else None)
         return (k, du) if want_du else k
 
     def step(self, u, dt, nsteps, record_ke=True):
         u = np.array(u, dtype=np.float64, order="C", copy=True)
         ke = np.zeros(nsteps) if record_ke else None
-        rc = lib().orc_step(self._h, _p(u), dt, nsteps, _p(ke) if record_ke else None)
+        rc = self._lib.orc_step(self._h, _p(u), dt, nsteps, _p(ke) if record_ke else None)
         if rc != 0:
             raise MemoryError("oracle step failed")
         return u, ke
 
     def ke(self, u):
         u = np.ascontiguousarray(u, dtype=np.float64)
-        return lib().orc_ke(self._h, _p(u))
+        return self._lib.orc_ke(self._h, _p(u))
 
     def diagnostics(self, u):
         u = np.ascontiguousarray(u, dtype=np.float64)
         out = np.zeros(2)
-        lib().orc_diagnostics(self._h, _p(u), _p(out))
+        self._lib.orc_diagnostics(self._h, _p(u), _p(out))
         return out
 
     def dt(self, cfl):
-        return lib().orc_dt(self._h, cfl)
+        return self._lib.orc_dt(self._h, cfl)
 
     def gauss(self):
         x = np.zeros(self.n)
         w = np.zeros(self.n)
-        lib().orc_get_gauss(self._h, _p(x), _p(w))
+        self._lib.orc_get_gauss(self._h, _p(x), _p(w))
         return x, w
 
     def coords(self):
         c = np.zeros((self.K, self.dofs, 3))
-        lib().orc_get_coords(self._h, _p(c))
+        self._lib.orc_get_coords(self._h, _p(c))
         return c
 
     def operator(self, which):
@@ -166,7 +170,7 @@ class Oracle:
         shapes = {0: (n * n, n * n), 1: (d, d), 2: (d, d), 3: (3, d, d), 4: (3, d, d), 5: (3, n * n, n * n)}
         w = names[which]
         out = np.zeros(int(np.prod(shapes[w])))
-        lib().orc_get_operator(self._h, w, _p(out))
+        self._lib.orc_get_operator(self._h, w, _p(out))
         # column-major (rows fastest) -> numpy [.., col, row] then transpose last two
         sh = shapes[w]
         if len(sh) == 2:
@@ -175,6 +179,6 @@ class Oracle:
 
     def physics(self):
         out = np.zeros(11)
-        lib().orc_get_physics(self._h, _p(out))
+        self._lib.orc_get_physics(self._h, _p(out))
         keys = ["Re", "Pr", "gamma", "M0", "rho0", "V0", "R", "c0", "p0", "T0", "J"]
         return dict(zip(keys, out))

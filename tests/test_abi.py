@@ -1,0 +1,62 @@
+"""CPU-side checks of the C ABI: the library loads, exports every symbol
+include/ldg.h declares, and rejects bad configurations before touching a
+device (no compute calls here)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+import ldg_amd
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    text = open(os.path.join(ROOT, "include", "ldg.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(ldg_[a-z_]+)\s*\(", text)))
+
+
+def test_header_matches_binding_list():
+    assert declared_symbols() == sorted(ldg_amd.EXPORTED)
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ldg_amd.load_library()
+    for name in declared_symbols():
+        assert hasattr(lib, name), name
+
+
+def test_config_default_is_reference_defaults():
+    lib = ldg_amd.load_library()
+    cfg = ldg_amd.ldg_config()
+    assert lib.ldg_config_default(ctypes.byref(cfg)) == 0
+    assert (cfg.p, cfg.nx, cfg.ny, cfg.nz) == (2, 10, 10, 10)
+    assert (cfg.Re, cfg.Pr, cfg.gamma, cfg.M0, cfg.rho0, cfg.V0, cfg.L, cfg.R) == (1600.0, 0.71, 1.4, 0.1, 1.0, 1.0, 1.0, 8.314)
+    assert cfg.xmin[0] == pytest.approx(-3.141592653589793) and cfg.xmax[2] == pytest.approx(3.141592653589793)
+
+
+@pytest.mark.parametrize("field,value", [("p", 0), ("p", 6), ("nx", 2), ("nz", 1), ("Re", 0.0), ("gamma", 1.0)])
+def test_create_rejects_bad_config(field, value):
+    lib = ldg_amd.load_library()
+    cfg = ldg_amd.ldg_config()
+    lib.ldg_config_default(ctypes.byref(cfg))
+    setattr(cfg, field, value)
+    h = ctypes.c_void_p()
+    rc = lib.ldg_create(ctypes.byref(cfg), ctypes.byref(h))
+    assert rc == ldg_amd.LDG_EINVAL
+    assert not h.value
+    assert len(lib.ldg_last_error(None)) > 0
+
+
+def test_null_arguments_are_rejected():
+    lib = ldg_amd.load_library()
+    assert lib.ldg_create(None, None) == ldg_amd.LDG_EINVAL
+    assert lib.ldg_sizes(None, None, None) == ldg_amd.LDG_EINVAL
+    assert lib.ldg_step(None, 0.1, 1, None) == ldg_amd.LDG_EINVAL
+
+
+def test_mesh_rejects_thin_mesh():
+    with pytest.raises(ValueError):
+        ldg_amd.Mesh(2, 8, 8)
