@@ -50,7 +50,7 @@ def main():
     ap.add_argument("--p", type=int, default=4)
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-nx", type=int, default=16)
+    ap.add_argument("--cpu-nx", type=int, default=32)
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (HBM counter calibration) and exit")

@@ -205,6 +205,8 @@ __device__ inline double block_sum(double v, double *sred) {
 // ---------------------------------------------------------------------
 // Kernel A: auxiliary gradient (solver.cpp:490-508, restated in strong
 // form: -M1^{-1} S1 + face lifts = Dn + M1^{-1} e_0 e_0^T, exactly).
+// One thread per node computes all 15 components and stores them straight
+// to HBM (consecutive threads -> consecutive nodes, coalesced).
 // ---------------------------------------------------------------------
 template <int N, int E>
 __global__ void __launch_bounds__(NT) ldg_grad_kernel(const double *__restrict__ U, double *__restrict__ Q,
@@ -212,13 +214,17 @@ __global__ void __launch_bounds__(NT) ldg_grad_kernel(const double *__restrict__
   constexpr int NN = N * N * N, NF = N * N;
   __shared__ double su[E][NSTATE][NN];
   __shared__ double sj[E][3][NSTATE][NF];
-  __shared__ double sq[E][NGRAD][NN];
+  __shared__ double sD[N][N + 1];
   const int lb = logical_block(m);
   if (lb < 0) return;
   const long k0 = (long)lb * E;
   const int ne = (int)((m.nelem - k0) < E ? (m.nelem - k0) : E);
   const int tid = threadIdx.x;
 
+  if (tid < N * (N + 1)) {
+    const int i = tid / (N + 1), j = tid % (N + 1);
+    sD[i][j] = j < N ? op.Dn[i][j] : op.L0[i];
+  }
   const double *Ub = U + k0 * NSTATE * NN;
   for (int w = tid; w < ne * NSTATE * NN; w += NT) (&su[0][0][0])[w] = Ub[w];
   __syncthreads();
@@ -238,45 +244,56 @@ __global__ void __launch_bounds__(NT) ldg_grad_kernel(const double *__restrict__
   }
   __syncthreads();
 
-  for (int w = tid; w < ne * 3 * NSTATE * NF; w += NT) {
-    const int e = w / (3 * NSTATE * NF);
-    const int r = w - e * (3 * NSTATE * NF);
-    const int d = r / (NSTATE * NF);
-    const int s = (r / NF) % NSTATE;
-    const int ab = r % NF;
-    const int a = ab % N, b = ab / N;
-    const int base = line_base<N>(d, a, b), st = line_stride<N>(d);
-    double v[N];
+  for (int w = tid; w < ne * NN; w += NT) {
+    const int e = w / NN, node = w - e * NN;
+    const int ix = node % N, iy = (node / N) % N, iz = node / NF;
+    double rx[N + 1], ry[N + 1], rz[N + 1];
 #pragma unroll
-    for (int i = 0; i < N; ++i) v[i] = su[e][s][base + i * st];
-    const double jmp = sj[e][d][s][ab];
-    const double ia = m.ialpha[d];
+    for (int j = 0; j <= N; ++j) { rx[j] = sD[ix][j]; ry[j] = sD[iy][j]; rz[j] = sD[iz][j]; }
+    double *Qe = Q + (k0 + e) * NGRAD * NN + node;
 #pragma unroll
-    for (int i = 0; i < N; ++i) {
-      double acc = op.L0[i] * jmp;
+    for (int s = 0; s < NSTATE; ++s) {
+      const double *us = su[e][s];
+      double qx = rx[N] * sj[e][0][s][iy + N * iz];
+      double qy = ry[N] * sj[e][1][s][ix + N * iz];
+      double qz = rz[N] * sj[e][2][s][ix + N * iy];
 #pragma unroll
-      for (int j = 0; j < N; ++j) acc += op.Dn[i][j] * v[j];
-      sq[e][d * NSTATE + s][base + i * st] = acc * ia;
+      for (int j = 0; j < N; ++j) {
+        qx += rx[j] * us[j + N * iy + NF * iz];
+        qy += ry[j] * us[ix + N * j + NF * iz];
+        qz += rz[j] * us[ix + N * iy + NF * j];
+      }
+      Qe[(0 * NSTATE + s) * NN] = qx * m.ialpha[0];
+      Qe[(1 * NSTATE + s) * NN] = qy * m.ialpha[1];
+      Qe[(2 * NSTATE + s) * NN] = qz * m.ialpha[2];
     }
   }
-  __syncthreads();
-
-  double *Qb = Q + k0 * NGRAD * NN;
-  for (int w = tid; w < ne * NGRAD * NN; w += NT) Qb[w] = (&sq[0][0][0])[w];
 }
 
 // ---------------------------------------------------------------------
 // Kernel B: face fluxes + volume integrals + inverse mass + RK epilogue.
+//
+// LDS per element: a 20-field volume buffer (which first serves as the
+// face buffer of one direction: 2 sides x 25 fields x N^2 points) and the
+// face residual (5 x N^3).
 // ---------------------------------------------------------------------
+template <int N>
+struct RhsLds {
+  static constexpr int NN = N * N * N, NF = N * N;
+  static constexpr int NV = NSTATE + NGRAD;          // 20 volume fields
+  static constexpr int NFF = 25;                     // face fields: uK(5), uN(5), q~(15)
+  static constexpr int SV = NV * NN;
+  static constexpr int FB = 2 * NFF * NF;
+  static constexpr int BUF = SV > FB ? SV : FB;      // per element
+};
+
 template <int N, int E>
 __global__ void __launch_bounds__(NT) ldg_rhs_kernel(const double *__restrict__ U, const double *__restrict__ Q,
                                                        MeshArgs m, Ops<N, double> op, PhysArgs ph, StageArgs st) {
-  constexpr int NN = N * N * N, NF = N * N;
-  constexpr int NV = NSTATE + NGRAD;   // 20 fields per element
-  constexpr int NFF = 25;              // face fields: uK(5), uN(5), q~(15)
-  __shared__ double sv[E][NV][NN];
+  using L = RhsLds<N>;
+  constexpr int NN = L::NN, NF = L::NF, NV = L::NV, NFF = L::NFF, BUF = L::BUF;
+  __shared__ double sbuf[E][BUF];
   __shared__ double sr[E][NSTATE][NN];
-  __shared__ double sf[E][2][NFF][NF];
   __shared__ double sred[NT];
   const int lb = logical_block(m);
   if (lb < 0) return;
@@ -284,102 +301,93 @@ __global__ void __launch_bounds__(NT) ldg_rhs_kernel(const double *__restrict__ 
   const int ne = (int)((m.nelem - k0) < E ? (m.nelem - k0) : E);
   const int tid = threadIdx.x;
 
-  // ---- load U and Q; clear the face residual
-  {
-    const double *Ub = U + k0 * NSTATE * NN;
-    const double *Qb = Q + k0 * NGRAD * NN;
-    for (int w = tid; w < ne * NSTATE * NN; w += NT) {
-      const int e = w / (NSTATE * NN), r = w - e * (NSTATE * NN);
-      sv[e][0][r] = Ub[w];
-      (&sr[0][0][0])[w] = 0.0;
-    }
-    for (int w = tid; w < ne * NGRAD * NN; w += NT) {
-      const int e = w / (NGRAD * NN), r = w - e * (NGRAD * NN);
-      sv[e][NSTATE][r] = Qb[w];
-    }
-  }
-  __syncthreads();
+  for (int w = tid; w < ne * NSTATE * NN; w += NT) (&sr[0][0][0])[w] = 0.0;
 
-  // ---- faces, one direction at a time (both sides of the element)
+  // ================= faces, one direction at a time =================
   for (int d = 0; d < 3; ++d) {
     const int st_d = line_stride<N>(d);
-    // gather: own trace, neighbour trace, and the gradient trace chosen by
-    // the one-sided switch (neighbour's on the positive face)
-    for (int w = tid; w < ne * 2 * NFF * NF; w += NT) {
-      const int e = w / (2 * NFF * NF);
-      const int r = w - e * (2 * NFF * NF);
-      const int side = r / (NFF * NF);
-      const int f = (r / NF) % NFF;
-      const int ab = r % NF;
-      const int a = ab % N, b = ab / N;
-      const int base = line_base<N>(d, a, b);
-      const int own = base + (side ? (N - 1) * st_d : 0);
-      const int opp = base + (side ? 0 : (N - 1) * st_d);
-      double val;
+    const int ast = d == 0 ? N : 1;          // memory stride of the face coordinate a
+    // gather nodal face data straight from HBM and interpolate along a:
+    // fb[side][f][qa + N b]; f = uK(0..4), uN(5..9), q~(10..24)
+    for (int w = tid; w < ne * 2 * NFF * N; w += NT) {
+      const int e = w / (2 * NFF * N);
+      const int r = w - e * (2 * NFF * N);
+      const int side = r / (NFF * N);
+      const int f = (r / N) % NFF;
+      const int b = r % N;
+      const long K = k0 + e;
+      const int own = side ? (N - 1) * st_d : 0;
+      const int opp = side ? 0 : (N - 1) * st_d;
+      const double *src;
       if (f < NSTATE) {
-        val = sv[e][f][own];
+        src = U + (K * NSTATE + f) * NN + own;
       } else if (f < 2 * NSTATE) {
-        const long Kn = neighbor(m, k0 + e, d, side ? 1 : -1);
-        val = U[(Kn * NSTATE + (f - NSTATE)) * NN + opp];
+        src = U + (neighbor(m, K, d, side ? 1 : -1) * NSTATE + (f - NSTATE)) * NN + opp;
       } else if (side == 0) {
-        val = sv[e][NSTATE + (f - 2 * NSTATE)][own];
+        src = Q + (K * NGRAD + (f - 2 * NSTATE)) * NN;                        // own, i_d = 0
       } else {
-        const long Kn = neighbor(m, k0 + e, d, 1);
-        val = Q[(Kn * NGRAD + (f - 2 * NSTATE)) * NN + opp];
+        src = Q + (neighbor(m, K, d, 1) * NGRAD + (f - 2 * NSTATE)) * NN;     // neighbour's i_d = 0
       }
-      sf[e][side][f][ab] = val;
+      src += line_base<N>(d, 0, b);
+      double v[N];
+#pragma unroll
+      for (int a = 0; a < N; ++a) v[a] = src[a * ast];
+      double *dst = &sbuf[e][(side * NFF + f) * NF + N * b];
+#pragma unroll
+      for (int q = 0; q < N; ++q) {
+        double acc = 0.0;
+#pragma unroll
+        for (int a = 0; a < N; ++a) acc += op.I[q][a] * v[a];
+        dst[q] = acc;
+      }
     }
     __syncthreads();
-    // face nodes -> face Gauss points, along a then along b (in place)
+    // interpolate along b (in place)
+    for (int w = tid; w < ne * 2 * NFF * N; w += NT) {
+      const int e = w / (2 * NFF * N);
+      const int r = w - e * (2 * NFF * N);
+      const int row = r / N;                  // side * NFF + f
+      const int qa = r % N;
+      double *p = &sbuf[e][row * NF + qa];
+      double v[N];
 #pragma unroll
-    for (int pass = 0; pass < 2; ++pass) {
-      const int lst = pass == 0 ? 1 : N;
-      for (int w = tid; w < ne * 2 * NFF * N; w += NT) {
-        const int e = w / (2 * NFF * N);
-        const int r = w - e * (2 * NFF * N);
-        const int side = r / (NFF * N);
-        const int f = (r / N) % NFF;
-        const int c = r % N;
-        const int base = pass == 0 ? c * N : c;
-        double *p = &sf[e][side][f][base];
-        double v[N];
+      for (int b = 0; b < N; ++b) v[b] = p[b * N];
 #pragma unroll
-        for (int i = 0; i < N; ++i) v[i] = p[i * lst];
+      for (int q = 0; q < N; ++q) {
+        double acc = 0.0;
 #pragma unroll
-        for (int q = 0; q < N; ++q) {
-          double acc = 0.0;
-#pragma unroll
-          for (int i = 0; i < N; ++i) acc += op.I[q][i] * v[i];
-          p[q * lst] = acc;
-        }
+        for (int b = 0; b < N; ++b) acc += op.I[q][b] * v[b];
+        p[q * N] = acc;
       }
-      __syncthreads();
     }
+    __syncthreads();
     // flux at each face Gauss point, weighted for the lift
     for (int w = tid; w < ne * 2 * NF; w += NT) {
       const int e = w / (2 * NF);
       const int r = w - e * (2 * NF);
       const int side = r / NF;
       const int ab = r % NF;
+      const double *fb = &sbuf[e][side * NFF * NF + ab];
       double uK[5], uN[5], qq[15], out[5];
 #pragma unroll
-      for (int s = 0; s < 5; ++s) { uK[s] = sf[e][side][s][ab]; uN[s] = sf[e][side][5 + s][ab]; }
+      for (int s = 0; s < 5; ++s) { uK[s] = fb[s * NF]; uN[s] = fb[(5 + s) * NF]; }
 #pragma unroll
-      for (int k = 0; k < 15; ++k) qq[k] = sf[e][side][10 + k][ab];
+      for (int k = 0; k < 15; ++k) qq[k] = fb[(10 + k) * NF];
       face_flux<double>(uK, uN, qq, d, side ? 1.0 : -1.0, ph, out);
       const double wt = op.w[ab % N] * op.w[ab / N] * m.ialpha[d];
+      double *o = &sbuf[e][side * NFF * NF + ab];
 #pragma unroll
-      for (int s = 0; s < 5; ++s) sf[e][side][s][ab] = out[s] * wt;
+      for (int s = 0; s < 5; ++s) o[s * NF] = out[s] * wt;
     }
     __syncthreads();
-    // lift: I^T along a (in place), then I^T along b straight into sr
+    // lift: I^T along a (in place)
     for (int w = tid; w < ne * 2 * NSTATE * N; w += NT) {
       const int e = w / (2 * NSTATE * N);
       const int r = w - e * (2 * NSTATE * N);
       const int side = r / (NSTATE * N);
       const int s = (r / N) % NSTATE;
-      const int c = r % N;
-      double *p = &sf[e][side][s][c * N];
+      const int qb = r % N;
+      double *p = &sbuf[e][(side * NFF + s) * NF + N * qb];
       double v[N];
 #pragma unroll
       for (int q = 0; q < N; ++q) v[q] = p[q];
@@ -392,37 +400,62 @@ __global__ void __launch_bounds__(NT) ldg_rhs_kernel(const double *__restrict__ 
       }
     }
     __syncthreads();
+    // I^T along b, subtracted into the face nodes of the residual
     for (int w = tid; w < ne * 2 * NSTATE * N; w += NT) {
       const int e = w / (2 * NSTATE * N);
       const int r = w - e * (2 * NSTATE * N);
       const int side = r / (NSTATE * N);
       const int s = (r / N) % NSTATE;
       const int a = r % N;
+      const double *p = &sbuf[e][(side * NFF + s) * NF + a];
       double v[N];
 #pragma unroll
-      for (int q = 0; q < N; ++q) v[q] = sf[e][side][s][a + q * N];
+      for (int q = 0; q < N; ++q) v[q] = p[q * N];
+      const int off = side ? (N - 1) * st_d : 0;
 #pragma unroll
       for (int b = 0; b < N; ++b) {
         double acc = 0.0;
 #pragma unroll
         for (int q = 0; q < N; ++q) acc += op.I[q][b] * v[q];
-        const int node = line_base<N>(d, a, b) + (side ? (N - 1) * st_d : 0);
-        sr[e][s][node] -= acc;
+        sr[e][s][line_base<N>(d, a, b) + off] -= acc;
       }
     }
     __syncthreads();
   }
 
-  // ---- volume: nodes -> Gauss points for all 20 fields (x, y, z sweeps)
+  // ================= volume =================
+  // load U and Q x-lines from HBM and interpolate along x
+  for (int w = tid; w < ne * NV * NF; w += NT) {
+    const int e = w / (NV * NF);
+    const int r = w - e * (NV * NF);
+    const int f = r / NF;
+    const int yz = r % NF;
+    const long K = k0 + e;
+    const double *src = f < NSTATE ? U + (K * NSTATE + f) * NN + N * yz
+                                   : Q + (K * NGRAD + (f - NSTATE)) * NN + N * yz;
+    double v[N];
 #pragma unroll
-  for (int dd = 0; dd < 3; ++dd) {
+    for (int i = 0; i < N; ++i) v[i] = src[i];
+    double *dst = &sbuf[e][f * NN + N * yz];
+#pragma unroll
+    for (int q = 0; q < N; ++q) {
+      double acc = 0.0;
+#pragma unroll
+      for (int i = 0; i < N; ++i) acc += op.I[q][i] * v[i];
+      dst[q] = acc;
+    }
+  }
+  __syncthreads();
+  // y then z sweeps (in place)
+#pragma unroll
+  for (int dd = 1; dd < 3; ++dd) {
     for (int w = tid; w < ne * NV * NF; w += NT) {
       const int e = w / (NV * NF);
       const int r = w - e * (NV * NF);
       const int f = r / NF;
       const int ab = r % NF;
-      double *p = &sv[e][f][line_base<N>(dd, ab % N, ab / N)];
       const int ls = line_stride<N>(dd);
+      double *p = &sbuf[e][f * NN + line_base<N>(dd, ab % N, ab / N)];
       double v[N];
 #pragma unroll
       for (int i = 0; i < N; ++i) v[i] = p[i * ls];
@@ -437,32 +470,32 @@ __global__ void __launch_bounds__(NT) ldg_rhs_kernel(const double *__restrict__ 
     __syncthreads();
   }
 
-  // ---- pointwise volume flux, weighted; optional Gauss diagnostics
+  // pointwise volume flux, weighted; optional Gauss diagnostics
   double dke = 0.0, dens = 0.0;
   for (int w = tid; w < ne * NN; w += NT) {
     const int e = w / NN, q = w - e * NN;
+    double *pv = &sbuf[e][q];
     double u[5], g[15], F[15];
 #pragma unroll
-    for (int s = 0; s < 5; ++s) u[s] = sv[e][s][q];
+    for (int s = 0; s < 5; ++s) u[s] = pv[s * NN];
 #pragma unroll
-    for (int k = 0; k < 15; ++k) g[k] = sv[e][NSTATE + k][q];
+    for (int k = 0; k < 15; ++k) g[k] = pv[(NSTATE + k) * NN];
+    const double wq = op.w[q % N] * op.w[(q / N) % N] * op.w[q / NF];
     if (st.diag_part) {
       const double ir = 1.0 / u[0];
       const double v[3] = {u[1] * ir, u[2] * ir, u[3] * ir};
       double gv[3][3], ge[3];
       prim_grads<double>(u, ir, v, g, gv, ge);
       const double w1 = gv[2][1] - gv[1][2], w2 = gv[0][2] - gv[2][0], w3 = gv[1][0] - gv[0][1];
-      const double wq = op.w[q % N] * op.w[(q / N) % N] * op.w[q / (N * N)];
       dke += wq * 0.5 * (u[1] * v[0] + u[2] * v[1] + u[3] * v[2]);
       dens += wq * 0.5 * u[0] * (w1 * w1 + w2 * w2 + w3 * w3);
     }
     volume_flux<double>(u, g, ph, F);
-    const double wq = op.w[q % N] * op.w[(q / N) % N] * op.w[q / (N * N)];
 #pragma unroll
     for (int l = 0; l < 3; ++l) {
       const double sc = wq * m.ialpha[l];
 #pragma unroll
-      for (int s = 0; s < 5; ++s) sv[e][NSTATE + 5 * l + s][q] = F[s + 5 * l] * sc;
+      for (int s = 0; s < 5; ++s) pv[(NSTATE + 5 * l + s) * NN] = F[s + 5 * l] * sc;
     }
   }
   if (st.diag_part) {
@@ -473,13 +506,13 @@ __global__ void __launch_bounds__(NT) ldg_rhs_kernel(const double *__restrict__ 
   }
   __syncthreads();
 
-  // ---- test-function contraction: z sweep (I^T on F_x, F_y; D^T on F_z)
+  // test-function contraction: z sweep (I^T on F_x, F_y; D^T on F_z)
   for (int w = tid; w < ne * NGRAD * NF; w += NT) {
     const int e = w / (NGRAD * NF);
     const int r = w - e * (NGRAD * NF);
     const int f = r / NF;            // 0..14 = s + 5 l
     const int ab = r % NF;
-    double *p = &sv[e][NSTATE + f][ab];
+    double *p = &sbuf[e][(NSTATE + f) * NN + ab];
     double v[N];
 #pragma unroll
     for (int q = 0; q < N; ++q) v[q] = p[q * NF];
@@ -508,10 +541,10 @@ __global__ void __launch_bounds__(NT) ldg_rhs_kernel(const double *__restrict__ 
     const int r = w - e * (NSTATE * NF);
     const int s = r / NF;
     const int ab = r % NF;
-    const int base = (ab % N) + N * N * (ab / N);
-    double *px = &sv[e][NSTATE + s][base];
-    double *py = &sv[e][NSTATE + 5 + s][base];
-    double *pz = &sv[e][NSTATE + 10 + s][base];
+    const int base = (ab % N) + NF * (ab / N);
+    double *px = &sbuf[e][(NSTATE + s) * NN + base];
+    double *py = &sbuf[e][(NSTATE + 5 + s) * NN + base];
+    const double *pz = &sbuf[e][(NSTATE + 10 + s) * NN + base];
     double vx[N], vy[N], vz[N];
 #pragma unroll
     for (int q = 0; q < N; ++q) { vx[q] = px[q * N]; vy[q] = py[q * N]; vz[q] = pz[q * N]; }
@@ -535,8 +568,8 @@ __global__ void __launch_bounds__(NT) ldg_rhs_kernel(const double *__restrict__ 
     const int s = r / NF;
     const int ab = r % NF;
     const int base = N * ab;
-    const double *px = &sv[e][NSTATE + s][base];
-    const double *pg = &sv[e][NSTATE + 5 + s][base];
+    const double *px = &sbuf[e][(NSTATE + s) * NN + base];
+    const double *pg = &sbuf[e][(NSTATE + 5 + s) * NN + base];
     double vx[N], vg[N], rr[N];
 #pragma unroll
     for (int q = 0; q < N; ++q) { vx[q] = px[q]; vg[q] = pg[q]; }
@@ -547,7 +580,7 @@ __global__ void __launch_bounds__(NT) ldg_rhs_kernel(const double *__restrict__ 
       for (int q = 0; q < N; ++q) acc += op.D[q][i] * vx[q] + op.I[q][i] * vg[q];
       rr[i] = acc;
     }
-    double *po = &sv[e][s][base];
+    double *po = &sbuf[e][s * NN + base];
 #pragma unroll
     for (int i = 0; i < N; ++i) {
       double acc = 0.0;
@@ -566,7 +599,7 @@ __global__ void __launch_bounds__(NT) ldg_rhs_kernel(const double *__restrict__ 
       const int s = r / NF;
       const int ab = r % NF;
       const int ls = line_stride<N>(dd);
-      double *p = &sv[e][s][line_base<N>(dd, ab % N, ab / N)];
+      double *p = &sbuf[e][s * NN + line_base<N>(dd, ab % N, ab / N)];
       double v[N];
 #pragma unroll
       for (int i = 0; i < N; ++i) v[i] = p[i * ls];
@@ -585,7 +618,7 @@ __global__ void __launch_bounds__(NT) ldg_rhs_kernel(const double *__restrict__ 
   const long gofs = k0 * NSTATE * NN;
   for (int w = tid; w < ne * NSTATE * NN; w += NT) {
     const int e = w / (NSTATE * NN), r = w - e * (NSTATE * NN);
-    const double k = sv[e][0][r];
+    const double k = sbuf[e][r];
     const long gi = gofs + w;
     if (st.kout) st.kout[gi] = k;
     if (st.acc || st.unext) {
@@ -593,7 +626,7 @@ __global__ void __launch_bounds__(NT) ldg_rhs_kernel(const double *__restrict__ 
       if (st.acc) {
         const double a = st.acc_init ? u0 + st.dtb * k : st.acc[gi] + st.dtb * k;
         st.acc[gi] = a;
-        if (st.ke_part) sv[e][NSTATE][r] = a;   // fields 5..9 are free again
+        if (st.ke_part) sbuf[e][NSTATE * NN + r] = a;   // fields 5..9 are free again
       }
       if (st.unext) st.unext[gi] = u0 + st.dta * k;
     }
@@ -603,15 +636,16 @@ __global__ void __launch_bounds__(NT) ldg_rhs_kernel(const double *__restrict__ 
     double kel = 0.0;
     for (int w = tid; w < ne * NN; w += NT) {
       const int e = w / NN, q = w - e * NN;
-      const double *a = &sv[e][NSTATE][0];
+      const double *a = &sbuf[e][NSTATE * NN];
       const double rho = a[q], m1 = a[NN + q], m2 = a[2 * NN + q], m3 = a[3 * NN + q];
-      const double wq = op.w[q % N] * op.w[(q / N) % N] * op.w[q / (N * N)];
+      const double wq = op.w[q % N] * op.w[(q / N) % N] * op.w[q / NF];
       kel += wq * (m1 * m1 + m2 * m2 + m3 * m3) / (2.0 * rho);
     }
     const double t = block_sum(kel, sred);
     if (tid == 0) st.ke_part[lb] = t;
   }
 }
+
 
 // Ordered sum of per-workgroup partials: out[j] = scale * sum_b part[b*stride + j].
 __global__ void __launch_bounds__(NT) ldg_finalize_kernel(const double *__restrict__ part, int nparts, int stride,

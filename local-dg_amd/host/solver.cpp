@@ -1,0 +1,102 @@
+#include "solver.hpp"
+
+#include <chrono>
+#include <cmath>
+#include <iostream>
+#include <stdexcept>
+#include <sys/stat.h>
+
+#include "io.hpp"
+
+namespace ldg {
+
+Solver::Solver() : Solver(2, 10, 1.0, 1.0, Mesh()) {}
+
+Solver::Solver(int p, int dtSnaps_, double tf_, double L, const Mesh &mesh_, const SolverOptions &opt_)
+    : mesh(mesh_), opt(opt_), tf(tf_), dtSnaps(dtSnaps_), order(p) {
+  ldg_config c;
+  ldg_config_default(&c);
+  c.p = p;
+  c.nx = mesh.nx; c.ny = mesh.ny; c.nz = mesh.nz;
+  c.xmin[0] = mesh.botLeft.x; c.xmin[1] = mesh.botLeft.y; c.xmin[2] = mesh.botLeft.z;
+  c.xmax[0] = mesh.topRight.x; c.xmax[1] = mesh.topRight.y; c.xmax[2] = mesh.topRight.z;
+  c.L = L;
+  c.Re = opt.Re; c.Pr = opt.Pr; c.gamma = opt.gamma; c.M0 = opt.M0; c.rho0 = opt.rho0; c.V0 = opt.V0; c.R = opt.R;
+  c.precision = opt.precision;
+  check(ldg_create(&c, &h), "ldg_create");
+  int nel = 0;
+  ldg_sizes(h, &dofs_, &nel);
+  // dt from the CFL rule, then rounded so that tf is hit exactly (solver.cpp:47-52)
+  dt = ldg_time_step(h, opt.cfl);
+  timesteps = (long)std::ceil(tf / dt);
+  dt = tf / timesteps;
+  check(ldg_initial_condition(h), "ldg_initial_condition");
+  // node coordinates for the snapshot files (mesh.cpp:389-408)
+  const int n = p + 1;
+  coords.realloc(3, dofs_, mesh.nElements);
+  const int ns[3] = {mesh.nx, mesh.ny, mesh.nz};
+  const double lo[3] = {mesh.botLeft.x, mesh.botLeft.y, mesh.botLeft.z};
+  const double hi[3] = {mesh.topRight.x, mesh.topRight.y, mesh.topRight.z};
+  for (long k = 0; k < mesh.nElements; ++k) {
+    const long ie[3] = {k % mesh.nx, (k / mesh.nx) % mesh.ny, k / ((long)mesh.nx * mesh.ny)};
+    for (int iN = 0; iN < dofs_; ++iN) {
+      const int in[3] = {iN % n, (iN / n) % n, iN / (n * n)};
+      for (int l = 0; l < 3; ++l) {
+        const double a = ie[l] * (hi[l] - lo[l]) / ns[l] + lo[l];
+        const double b = (ie[l] + 1) * (hi[l] - lo[l]) / ns[l] + lo[l];
+        coords(l, iN, k) = a + .5 * (std::cos((p - in[l]) * M_PI / p) + 1.0) * (b - a);
+      }
+    }
+  }
+}
+
+Solver::~Solver() { ldg_destroy(h); }
+
+void Solver::check(int rc, const char *what) const {
+  if (rc != LDG_OK) throw std::runtime_error(std::string(what) + ": " + ldg_last_error(h));
+}
+
+const darray &Solver::u() {
+  if (uHost.size() != (std::size_t)dofs_ * 5 * mesh.nElements) uHost.realloc(dofs_, 5, mesh.nElements);
+  check(ldg_get_state(h, uHost.data()), "ldg_get_state");
+  return uHost;
+}
+
+void Solver::dgTimeStep() {
+  kes.assign(timesteps, 0.0);
+  if (opt.verbose) std::cout << "Time stepping until tf = " << tf << std::endl;
+  if (opt.snapshots) mkdir(opt.outputDir.c_str(), 0755);
+  const auto start = std::chrono::steady_clock::now();
+  long iStep = 0;
+  while (iStep < timesteps) {
+    const double elapsed = std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();
+    if (elapsed > opt.wallLimitSeconds) {     // dump state and stop (solver.cpp:349-360)
+      const std::string name = opt.outputDir + "/xyznavier.tmp";
+      if (!initXYZVFile(name, 0, "u", 5) || !exportToXYZVFile(name, 0, coords, u()))
+        throw std::runtime_error("could not write " + name);
+      kes.resize(iStep);
+      break;
+    }
+    if (opt.snapshots && iStep % dtSnaps == 0) {
+      const std::string name = opt.outputDir + "/xyzu.txt";
+      if (!initXYZVFile(name, (int)(iStep / dtSnaps), "u", 5) ||
+          !exportToXYZVFile(name, (int)(iStep / dtSnaps), coords, u()))
+        throw std::runtime_error("could not write " + name);
+    }
+    const long chunk = std::min<long>(timesteps - iStep, dtSnaps - iStep % dtSnaps);
+    if (opt.verbose) std::cout << "time = " << iStep * dt << std::endl;
+    check(ldg_step(h, dt, (int)chunk, kes.data() + iStep), "ldg_step");
+    iStep += chunk;
+  }
+  // dissipation -(KE[i] - KE[i-2]) / (2 dt) (solver.cpp:437-440)
+  keDiss.clear();
+  for (std::size_t i = 2; i < kes.size(); ++i) keDiss.push_back(-(kes[i] - kes[i - 2]) / (2 * dt));
+  const double elapsed = std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();
+  if (opt.verbose) {
+    std::cout << "Finished time stepping. Time elapsed = " << elapsed << std::endl;
+    std::cout << "Outputting kinetic energy dissipation for Navier-Stokes: " << std::endl;
+    for (std::size_t i = 0; i < keDiss.size(); ++i) std::cout << keDiss[i] << (i + 1 < keDiss.size() ? ", " : "\n");
+  }
+}
+
+}  // namespace ldg

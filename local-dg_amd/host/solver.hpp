@@ -1,0 +1,64 @@
+// solver.hpp -- the reference's Solver API (solver.h:20-115) on libldg.so.
+//
+//   Solver(int p, int dtSnaps, double tf, double L, const Mesh& mesh);
+//   void dgTimeStep();
+//
+// Construction builds the operators and the Taylor-Green state on the GPU
+// (Solver::Solver / initialCondition, solver.cpp:7-71, 194-236);
+// dgTimeStep() runs the RK4 loop to tf on the GPU (solver.cpp:295-453).
+// Options the reference hard-codes are gathered in SolverOptions with the
+// reference's values as defaults.
+#pragma once
+
+#include <string>
+#include <vector>
+
+#include "array.hpp"
+#include "ldg.h"
+#include "mesh.hpp"
+
+namespace ldg {
+
+struct SolverOptions {
+  double cfl = 0.01;              // solver.cpp:47
+  double Re = 1600.0, Pr = 0.71, gamma = 1.4, M0 = 0.1, rho0 = 1.0, V0 = 1.0, R = 8.314;  // solver.cpp:30-36
+  int precision = LDG_FP64;
+  bool snapshots = true;          // output/xyzu.txt.<n> every dtSnaps steps (solver.cpp:366-381)
+  std::string outputDir = "output";
+  double wallLimitSeconds = 19.5 * 60 * 60;   // solver.cpp:351
+  bool verbose = true;
+};
+
+class Solver {
+ public:
+  Solver();                                                                   // Solver{2, 10, 1.0, 1.0, Mesh{}}
+  Solver(int p, int dtSnaps, double tf, double L, const Mesh &mesh, const SolverOptions &opt = SolverOptions());
+  ~Solver();
+  Solver(const Solver &) = delete;
+  Solver &operator=(const Solver &) = delete;
+
+  void dgTimeStep();
+
+  // results and state
+  const std::vector<double> &kineticEnergy() const { return kes; }
+  const std::vector<double> &keDissipation() const { return keDiss; }
+  double timeStep() const { return dt; }
+  long timeSteps() const { return timesteps; }
+  int dofs() const { return dofs_; }
+  const darray &u();                      // current state (copied from the device)
+  const darray &globalCoords() const { return coords; }
+  ldg_handle *handle() { return h; }
+
+ private:
+  Mesh mesh;
+  SolverOptions opt;
+  ldg_handle *h = nullptr;
+  double tf, dt;
+  long timesteps;
+  int dtSnaps, order, dofs_;
+  darray uHost, coords;
+  std::vector<double> kes, keDiss;
+  void check(int rc, const char *what) const;
+};
+
+}  // namespace ldg

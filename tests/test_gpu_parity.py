@@ -166,3 +166,24 @@ def test_step_rejects_bad_dt_and_reports_nonfinite():
     with pytest.raises(ldg_amd.LDGError) as e:
         g.step(0.01, 1)
     assert e.value.code == ldg_amd.LDG_ENONFINITE
+
+
+def test_cpp_driver_runs_and_writes_snapshots(tmp_path):
+    """The C++ host (Mesh / Solver / dgTimeStep over the C ABI) runs the
+    Taylor-Green case end to end and writes reference-format snapshots."""
+    import subprocess
+    import os
+    exe = os.path.join(os.path.dirname(ldg_amd.LIB_PATH), "ldg_driver")
+    out = tmp_path / "output"
+    r = subprocess.run([exe, "-p", "2", "-nx", "4", "-tf", "0.02", "-dtSnaps", "5", "-out", str(out)],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "Outputting kinetic energy dissipation" in r.stdout
+    snap = (out / "xyzu.txt.0").read_text().splitlines()
+    assert snap[0] == "X, Y, Z, u0, u1, u2, u3, u4"
+    assert len(snap) == 1 + 27 * 64
+    o = O.Oracle(2, 4)
+    u0 = o.initial_condition()
+    vals = np.array([[float(v) for v in line.split(",")] for line in snap[1:]])
+    np.testing.assert_allclose(vals[:, 3:].reshape(64, 27, 5), u0.transpose(0, 2, 1), rtol=1e-5, atol=1e-6)
+    assert (out / "xyzu.txt.1").exists()
