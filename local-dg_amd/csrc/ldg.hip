@@ -6,7 +6,8 @@
 //   acc    u^n + dt sum_s b_s k_s, becomes u^{n+1} after stage 4
 //   ua/ub  stage inputs U_s (double-buffered: stage s+1 is written while
 //          neighbours of stage s are still being read)
-//   q      LDG auxiliary gradient (15 fields per element)
+//   q      LDG auxiliary gradient (15 fields per element; FP64, or FP32 in
+//          the mixed mode)
 #include "ldg.h"
 
 #include <hip/hip_runtime.h>
@@ -44,8 +45,11 @@ struct ldg_handle {
   double dx[3]{};
   Basis1D basis;
   hipStream_t stream = nullptr;
-  double *u = nullptr, *acc = nullptr, *ua = nullptr, *ub = nullptr, *q = nullptr, *k = nullptr;
+  double *u = nullptr, *acc = nullptr, *ua = nullptr, *ub = nullptr, *k = nullptr;
+  void *q = nullptr;     // gradient: double, or float when mixed
+  bool mixed = false;
   double *ke_part = nullptr, *diag_part = nullptr, *ke_hist = nullptr, *scal = nullptr;
+  void *ops = nullptr;   // Ops<N, double> in device memory
   int nblocks_rhs = 0, nblocks_grad = 0;
   size_t ke_hist_cap = 0;
   std::string err;
@@ -98,7 +102,10 @@ PhysArgs phys_args(const ldg_handle *h) {
   p.mu = 1.0 / h->cfg.Re;
   p.kappa = p.mu * h->cfg.gamma / h->cfg.Pr;
   const double c0 = h->cfg.V0 / h->cfg.M0;
-  p.pref = h->cfg.rho0 * c0 * c0 / h->cfg.gamma;   // solver.cpp:40-41
+  const double p0 = h->cfg.rho0 * c0 * c0 / h->cfg.gamma;   // solver.cpp:40-41
+  p.rho0 = h->cfg.rho0;
+  p.E0 = p0 / p.gm1;
+  p.H0 = p.E0 + p0;
   return p;
 }
 
@@ -107,7 +114,11 @@ int launch_grad_t(ldg_handle *h, const double *U) {
   constexpr int E = Tile<N>::EG;
   MeshArgs m = mesh_args(h, E);
   dim3 grid(8 * m.per_xcd), block(NT);
-  hipLaunchKernelGGL((ldg_grad_kernel<N, E>), grid, block, 0, h->stream, U, h->q, m, make_ops<N>(h->basis));
+  const Ops<N, double> *op = (const Ops<N, double> *)h->ops;
+  if (h->mixed)
+    hipLaunchKernelGGL((ldg_grad_kernel<N, E, float>), grid, block, 0, h->stream, U, (float *)h->q, m, op);
+  else
+    hipLaunchKernelGGL((ldg_grad_kernel<N, E, double>), grid, block, 0, h->stream, U, (double *)h->q, m, op);
   HIPCHECK(h, hipGetLastError());
   return LDG_OK;
 }
@@ -117,8 +128,13 @@ int launch_rhs_t(ldg_handle *h, const double *U, const StageArgs &st) {
   constexpr int E = Tile<N>::E;
   MeshArgs m = mesh_args(h, E);
   dim3 grid(8 * m.per_xcd), block(NT);
-  hipLaunchKernelGGL((ldg_rhs_kernel<N, E>), grid, block, 0, h->stream, U, (const double *)h->q, m,
-                     make_ops<N>(h->basis), phys_args(h), st);
+  const Ops<N, double> *op = (const Ops<N, double> *)h->ops;
+  if (h->mixed)
+    hipLaunchKernelGGL((ldg_rhs_kernel<N, E, float>), grid, block, 0, h->stream, U, (const float *)h->q, m, op,
+                       phys_args(h), st);
+  else
+    hipLaunchKernelGGL((ldg_rhs_kernel<N, E, double>), grid, block, 0, h->stream, U, (const double *)h->q, m, op,
+                       phys_args(h), st);
   HIPCHECK(h, hipGetLastError());
   return LDG_OK;
 }
@@ -141,6 +157,25 @@ int launch_rhs(ldg_handle *h, const double *U, const StageArgs &st) {
     case 4: return launch_rhs_t<4>(h, U, st);
     case 5: return launch_rhs_t<5>(h, U, st);
     case 6: return launch_rhs_t<6>(h, U, st);
+  }
+  return fail(h, LDG_EINVAL, "unsupported order");
+}
+
+template <int N>
+int upload_ops_t(ldg_handle *h) {
+  const Ops<N, double> o = make_ops<N>(h->basis);
+  HIPCHECK(h, hipMalloc(&h->ops, sizeof(o)));
+  HIPCHECK(h, hipMemcpy(h->ops, &o, sizeof(o), hipMemcpyHostToDevice));
+  return LDG_OK;
+}
+
+int upload_ops(ldg_handle *h) {
+  switch (h->n) {
+    case 2: return upload_ops_t<2>(h);
+    case 3: return upload_ops_t<3>(h);
+    case 4: return upload_ops_t<4>(h);
+    case 5: return upload_ops_t<5>(h);
+    case 6: return upload_ops_t<6>(h);
   }
   return fail(h, LDG_EINVAL, "unsupported order");
 }
@@ -219,7 +254,7 @@ int ldg_create(const ldg_config *cfg, ldg_handle **out) {
   if (!(cfg->Re > 0) || !(cfg->Pr > 0) || !(cfg->gamma > 1) || !(cfg->M0 > 0) || !(cfg->rho0 > 0) ||
       !(cfg->V0 > 0) || !(cfg->L > 0) || !(cfg->R > 0))
     return fail(nullptr, LDG_EINVAL, "physics parameters must be positive (gamma > 1)");
-  if (cfg->precision != LDG_FP64) return fail(nullptr, LDG_EINVAL, "precision mode not available in this build");
+  if (cfg->precision != LDG_FP64 && cfg->precision != LDG_MIXED) return fail(nullptr, LDG_EINVAL, "unknown precision mode");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(nullptr, LDG_EDEVICE, "no HIP device");
 
@@ -232,6 +267,7 @@ int ldg_create(const ldg_config *cfg, ldg_handle **out) {
   const int ns[3] = {cfg->nx, cfg->ny, cfg->nz};
   for (int l = 0; l < 3; ++l) h->dx[l] = (cfg->xmax[l] - cfg->xmin[l]) / ns[l];
   h->basis = make_basis(cfg->p);
+  h->mixed = cfg->precision == LDG_MIXED;
   h->nblocks_rhs = rhs_blocks(h->n, h->K);
   auto bad = [&](int code) { int c = code; g_create_error = h->err; ldg_destroy(h); return c; };
   if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
@@ -242,12 +278,18 @@ int ldg_create(const ldg_config *cfg, ldg_handle **out) {
   double **arrs[5] = {&h->u, &h->acc, &h->ua, &h->ub, &h->k};
   for (double **p : arrs)
     if (hipMalloc((void **)p, bytes) != hipSuccess) { h->err = "device allocation failed (state)"; return bad(LDG_ENOMEM); }
-  if (hipMalloc((void **)&h->q, bytes * 3) != hipSuccess) { h->err = "device allocation failed (gradient)"; return bad(LDG_ENOMEM); }
+  const size_t qbytes = h->nU * 3 * (h->mixed ? sizeof(float) : sizeof(double));
+  if (hipMalloc(&h->q, qbytes) != hipSuccess) { h->err = "device allocation failed (gradient)"; return bad(LDG_ENOMEM); }
   if (hipMalloc((void **)&h->ke_part, sizeof(double) * (h->nblocks_rhs + 1)) != hipSuccess ||
       hipMalloc((void **)&h->diag_part, sizeof(double) * 2 * (h->nblocks_rhs + 1)) != hipSuccess ||
       hipMalloc((void **)&h->scal, sizeof(double) * 8) != hipSuccess) {
     h->err = "device allocation failed (partials)";
     return bad(LDG_ENOMEM);
+  }
+  if (upload_ops(h) != LDG_OK) {
+    g_create_error = h->err;
+    ldg_destroy(h);
+    return LDG_EDEVICE;
   }
   if (hipMemsetAsync(h->u, 0, bytes, h->stream) != hipSuccess || hipStreamSynchronize(h->stream) != hipSuccess) {
     h->err = "device initialisation failed";
@@ -260,8 +302,10 @@ int ldg_create(const ldg_config *cfg, ldg_handle **out) {
 void ldg_destroy(ldg_handle *h) {
   if (!h) return;
   if (h->stream) (void)hipStreamSynchronize(h->stream);
-  double *ptrs[] = {h->u, h->acc, h->ua, h->ub, h->q, h->k, h->ke_part, h->diag_part, h->ke_hist, h->scal};
+  double *ptrs[] = {h->u, h->acc, h->ua, h->ub, h->k, h->ke_part, h->diag_part, h->ke_hist, h->scal};
   for (double *p : ptrs) if (p) (void)hipFree(p);
+  if (h->q) (void)hipFree(h->q);
+  if (h->ops) (void)hipFree(h->ops);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
 }
@@ -329,8 +373,15 @@ int ldg_rhs(ldg_handle *h, const double *u, double *k, double *Du) {
   HIPCHECK(h, hipMemcpyAsync(k, h->k, bytes, hipMemcpyDeviceToHost, h->stream));
   if (Du) {
     std::vector<double> qh(h->nU * 3);
-    HIPCHECK(h, hipMemcpyAsync(qh.data(), h->q, bytes * 3, hipMemcpyDeviceToHost, h->stream));
-    HIPCHECK(h, hipStreamSynchronize(h->stream));
+    if (h->mixed) {
+      std::vector<float> qf(h->nU * 3);
+      HIPCHECK(h, hipMemcpyAsync(qf.data(), h->q, h->nU * 3 * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+      HIPCHECK(h, hipStreamSynchronize(h->stream));
+      for (size_t i = 0; i < qf.size(); ++i) qh[i] = qf[i];
+    } else {
+      HIPCHECK(h, hipMemcpyAsync(qh.data(), h->q, bytes * 3, hipMemcpyDeviceToHost, h->stream));
+      HIPCHECK(h, hipStreamSynchronize(h->stream));
+    }
     // (node, s + 5 l, element) -> Dus(node, s, element, l)
     const size_t dofs = h->dofs;
     for (long e = 0; e < h->K; ++e)

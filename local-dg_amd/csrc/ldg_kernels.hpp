@@ -11,9 +11,19 @@
 // and read the periodic neighbours' face slices straight from HBM (the
 // on-device form of the single-rank halo exchange, solver.cpp:929-998).
 //
-// Data in HBM (all FP64):
-//   U, u0, acc, unext : (node, state, element)  -- the reference layout
-//   Q                 : (node, s + 5*l, element) -- 15 gradient fields per element
+// Precision: the state, the RK4 combination, the inverse mass and every
+// reduction are FP64.  The interpolation, flux evaluation and test-function
+// contractions run in the compute type T (double for the plain path, float
+// for the mixed path), on fluctuations about the constant background
+// (rho0, 0, 0, 0, E0 = p0/(gamma-1)): the background is subtracted in FP64
+// when U is loaded (exact, both values are close), so the O(1/M0^2) part of
+// the energy never enters the T-precision arithmetic.  A constant state is a
+// discrete steady state, so the fluctuation form changes nothing but
+// rounding.
+//
+// Data in HBM:
+//   U, u0, acc, unext : FP64 (node, state, element) -- the reference layout
+//   Q                 : TQ (node, s + 5*l, element)  -- 15 gradient fields per element
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -24,8 +34,7 @@ constexpr int NSTATE = 5;
 constexpr int NGRAD = 15;
 constexpr int NT = 256;          // threads per workgroup
 
-// 1D operators (row-major [row][col]); passed by value as kernel arguments
-// so that fully unrolled loops read them as uniform scalars.
+// 1D operators (row-major [row][col]), FP64, in device memory.
 template <int N, typename T>
 struct Ops {
   T I[N][N];      // I[q][i]  = l_i(g_q)
@@ -46,7 +55,7 @@ struct MeshArgs {
 
 struct PhysArgs {
   double gamma, gm1, mu, kappa;   // kappa = mu gamma / Pr acting on grad(e)
-  double pref;                    // background pressure p0 = rho0 c0^2 / gamma
+  double rho0, E0, H0;            // background: E0 = p0/(gamma-1), H0 = E0 + p0
 };
 
 struct StageArgs {
@@ -77,6 +86,17 @@ __device__ inline long neighbor(const MeshArgs &m, long K, int d, int dir) {
   return ex + m.nx * (ey + (long)m.ny * ez);
 }
 
+// Face neighbours of the block's elements, computed once per workgroup
+// (the index arithmetic involves 64-bit division): nb[e][f] for faces
+// f = -x,+x,-y,+y,-z,+z.
+template <int E>
+__device__ inline void fill_neighbors(const MeshArgs &m, long k0, int ne, long (*nb)[6], int tid) {
+  if (tid < ne * 6) {
+    const int e = tid / 6, f = tid % 6;
+    nb[e][f] = neighbor(m, k0 + e, f >> 1, (f & 1) ? 1 : -1);
+  }
+}
+
 // node index of position i along direction d in the line with face
 // coordinates (a, b): x-lines (iy, iz), y-lines (ix, iz), z-lines (ix, iy)
 // -- the same (a, b) order as the reference's face map (mesh.cpp:428-501).
@@ -88,108 +108,116 @@ template <int N>
 __device__ inline int line_stride(int d) { return d == 0 ? 1 : d == 1 ? N : N * N; }
 
 // ---------------------------------------------------------------------
-// Pointwise physics (FP64), textbook forms of solver.cpp:1040-1374
+// Pointwise physics in fluctuation form, textbook forms of
+// solver.cpp:1040-1374.  A state w = (rho - rho0, rho u, rho v, rho w,
+// E - E0); p' = p - p0 = (gamma-1)(w4 - |m|^2/(2 rho)) and
+// E + p = H0 + w4 + p'.
 // ---------------------------------------------------------------------
+template <typename T>
+struct Bg {
+  T rho0, E0, H0, gm1, mu, kappa;
+  __device__ inline explicit Bg(const PhysArgs &p)
+      : rho0(T(p.rho0)), E0(T(p.E0)), H0(T(p.H0)), gm1(T(p.gm1)), mu(T(p.mu)), kappa(T(p.kappa)) {}
+};
 
 // Velocity gradient gv[i][l] and internal-energy gradient ge[l] from the
-// state and the conserved gradients g[s + 5 l].
+// state (density rho = 1/ir, velocity v, total energy E) and the conserved
+// gradients g[s + 5 l].
 template <typename T>
-__device__ inline void prim_grads(const T u[5], T ir, const T v[3], const T *g, T gv[3][3], T ge[3]) {
+__device__ inline void prim_grads(T ir, T E, const T v[3], const T *g, T gv[3][3], T ge[3]) {
 #pragma unroll
   for (int l = 0; l < 3; ++l) {
     const T drho = g[5 * l];
 #pragma unroll
     for (int i = 0; i < 3; ++i) gv[i][l] = (g[1 + i + 5 * l] - v[i] * drho) * ir;
-    const T dEr = (g[4 + 5 * l] - u[4] * ir * drho) * ir;
+    const T dEr = (g[4 + 5 * l] - E * ir * drho) * ir;
     ge[l] = dEr - (v[0] * gv[0][l] + v[1] * gv[1][l] + v[2] * gv[2][l]);
   }
 }
 
 // Volume flux F[s + 5 l] = Fc_l(u) - Fv_l(u, g) (fluxC minus fluxV,
-// solver.cpp:1040-1188).  The momentum flux carries p - pref instead of p:
-// a constant pressure contributes exactly nothing to the discrete residual
-// (the Gauss rules integrate d(phi)/dx and the face traces of phi exactly),
-// so this only removes the round-off of the O(1/M0^2) background.
+// solver.cpp:1040-1188).  The momentum flux carries p' = p - p0: a constant
+// pressure contributes exactly nothing to the discrete residual (the Gauss
+// rules integrate d(phi)/dx and the face traces of phi exactly).
 template <typename T>
-__device__ inline void volume_flux(const T u[5], const T *g, const PhysArgs &ph, T F[15]) {
-  const T ir = T(1) / u[0];
-  const T v[3] = {u[1] * ir, u[2] * ir, u[3] * ir};
-  const T pr = T(ph.gm1) * (u[4] - T(0.5) * (u[1] * v[0] + u[2] * v[1] + u[3] * v[2]));
-  const T pf = pr - T(ph.pref);
+__device__ inline void volume_flux(const T w[5], const T *g, const Bg<T> &b, T F[15]) {
+  const T rho = b.rho0 + w[0];
+  const T ir = T(1) / rho;
+  const T v[3] = {w[1] * ir, w[2] * ir, w[3] * ir};
+  const T pf = b.gm1 * (w[4] - T(0.5) * (w[1] * v[0] + w[2] * v[1] + w[3] * v[2]));
+  const T hv = b.H0 + w[4] + pf;   // E + p
   T gv[3][3], ge[3];
-  prim_grads<T>(u, ir, v, g, gv, ge);
-  const T mu = T(ph.mu), kap = T(ph.kappa);
+  prim_grads<T>(ir, b.E0 + w[4], v, g, gv, ge);
   const T div = gv[0][0] + gv[1][1] + gv[2][2];
 #pragma unroll
   for (int l = 0; l < 3; ++l) {
     T tau[3];
 #pragma unroll
-    for (int i = 0; i < 3; ++i) tau[i] = mu * (gv[i][l] + gv[l][i]) - (i == l ? mu * T(2.0 / 3.0) * div : T(0));
-    F[5 * l] = u[1 + l];
+    for (int i = 0; i < 3; ++i) tau[i] = b.mu * (gv[i][l] + gv[l][i]) - (i == l ? b.mu * T(2.0 / 3.0) * div : T(0));
+    F[5 * l] = w[1 + l];
 #pragma unroll
-    for (int i = 0; i < 3; ++i) F[1 + i + 5 * l] = u[1 + i] * v[l] + (i == l ? pf : T(0)) - tau[i];
-    F[4 + 5 * l] = (u[4] + pr) * v[l] - (v[0] * tau[0] + v[1] * tau[1] + v[2] * tau[2] + kap * ge[l]);
+    for (int i = 0; i < 3; ++i) F[1 + i + 5 * l] = w[1 + i] * v[l] + (i == l ? pf : T(0)) - tau[i];
+    F[4 + 5 * l] = hv * v[l] - (v[0] * tau[0] + v[1] * tau[1] + v[2] * tau[2] + b.kappa * ge[l]);
   }
 }
 
-// Face flux for element K on the face with outward normal sgn*e_d:
+// Face flux for element K on the face with outward normal sgn*e_D:
 // Roe(uN, uK, n) (numericalFluxC, solver.cpp:1194-1374) plus the one-sided
-// viscous flux -Fv_d(uK, q) sgn (numericalFluxV, solver.cpp:891-921; q is
+// viscous flux -Fv_D(uK, q) sgn (numericalFluxV, solver.cpp:891-921; q is
 // the neighbour's gradient on positive faces, the element's own otherwise).
-template <typename T>
-__device__ inline void face_flux(const T uK[5], const T uN[5], const T *q, int d, T sgn,
-                                 const PhysArgs &ph, T out[5]) {
-  const T gm1 = T(ph.gm1);
-  const T irN = T(1) / uN[0], irK = T(1) / uK[0];
-  const T vN[3] = {uN[1] * irN, uN[2] * irN, uN[3] * irN};
-  const T vK[3] = {uK[1] * irK, uK[2] * irK, uK[3] * irK};
-  const T pN = gm1 * (uN[4] - T(0.5) * (uN[1] * vN[0] + uN[2] * vN[1] + uN[3] * vN[2]));
-  const T pK = gm1 * (uK[4] - T(0.5) * (uK[1] * vK[0] + uK[2] * vK[1] + uK[3] * vK[2]));
-  const T HN = (uN[4] + pN) * irN, HK = (uK[4] + pK) * irK;
-  const T qN = sgn * vN[d], qK = sgn * vK[d];
-  const T s = sqrt(uN[0] * irK);
+template <typename T, int D>
+__device__ inline void face_flux(const T wK[5], const T wN[5], const T *q, T sgn, const Bg<T> &b, T out[5]) {
+  const T rN = b.rho0 + wN[0], rK = b.rho0 + wK[0];
+  const T irN = T(1) / rN, irK = T(1) / rK;
+  const T vN[3] = {wN[1] * irN, wN[2] * irN, wN[3] * irN};
+  const T vK[3] = {wK[1] * irK, wK[2] * irK, wK[3] * irK};
+  const T pN = b.gm1 * (wN[4] - T(0.5) * (wN[1] * vN[0] + wN[2] * vN[1] + wN[3] * vN[2]));   // p - p0
+  const T pK = b.gm1 * (wK[4] - T(0.5) * (wK[1] * vK[0] + wK[2] * vK[1] + wK[3] * vK[2]));
+  const T hN = b.H0 + wN[4] + pN, hK = b.H0 + wK[4] + pK;                                   // E + p
+  const T HN = hN * irN, HK = hK * irK;
+  const T qN = sgn * vN[D], qK = sgn * vK[D];
+  const T s = sqrt(rN * irK);
   const T iden = T(1) / (s + T(1));
   T vt[3];
 #pragma unroll
   for (int i = 0; i < 3; ++i) vt[i] = (s * vN[i] + vK[i]) * iden;
   const T ht = (s * HN + HK) * iden;
   const T q2 = vt[0] * vt[0] + vt[1] * vt[1] + vt[2] * vt[2];
-  const T qt = sgn * vt[d];
-  const T c2 = gm1 * (ht - T(0.5) * q2);
+  const T qt = sgn * vt[D];
+  const T c2 = b.gm1 * (ht - T(0.5) * q2);
   const T c = sqrt(c2), ic = T(1) / c;
   const T l0 = fabs(qt), lp = fabs(qt + c), lm = fabs(qt - c);
   T dd[5];
 #pragma unroll
-  for (int k = 0; k < 5; ++k) dd[k] = uN[k] - uK[k];
-  const T dp = gm1 * (T(0.5) * q2 * dd[0] - (vt[0] * dd[1] + vt[1] * dd[2] + vt[2] * dd[3]) + dd[4]);
-  const T dV = sgn * dd[1 + d] - qt * dd[0];
+  for (int k = 0; k < 5; ++k) dd[k] = wN[k] - wK[k];
+  const T dp = b.gm1 * (T(0.5) * q2 * dd[0] - (vt[0] * dd[1] + vt[1] * dd[2] + vt[2] * dd[3]) + dd[4]);
+  const T dV = sgn * dd[1 + D] - qt * dd[0];
   const T s1 = T(0.5) * (lp + lm) - l0, s2 = T(0.5) * (lp - lm);
   const T A = s1 * dp * ic * ic + s2 * dV * ic;
   const T B = s2 * dp * ic + s1 * dV;
-  out[0] = T(0.5) * (uN[0] * qN + uK[0] * qK) - T(0.5) * (l0 * dd[0] + A);
+  out[0] = T(0.5) * (rN * qN + rK * qK) - T(0.5) * (l0 * dd[0] + A);
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
-    const T pn = (i == d) ? sgn * ((pN - T(ph.pref)) + (pK - T(ph.pref))) : T(0);   // see volume_flux
-    const T bn = (i == d) ? sgn * B : T(0);
-    out[1 + i] = T(0.5) * (uN[1 + i] * qN + uK[1 + i] * qK + pn) - T(0.5) * (l0 * dd[1 + i] + A * vt[i] + bn);
+    const T pn = (i == D) ? sgn * (pN + pK) : T(0);
+    const T bn = (i == D) ? sgn * B : T(0);
+    out[1 + i] = T(0.5) * (wN[1 + i] * qN + wK[1 + i] * qK + pn) - T(0.5) * (l0 * dd[1 + i] + A * vt[i] + bn);
   }
-  out[4] = T(0.5) * ((uN[4] + pN) * qN + (uK[4] + pK) * qK) - T(0.5) * (l0 * dd[4] + A * ht + B * qt);
+  out[4] = T(0.5) * (hN * qN + hK * qK) - T(0.5) * (l0 * dd[4] + A * ht + B * qt);
 
-  // viscous part: only Fv_d is needed since n = sgn e_d
+  // viscous part: only Fv_D is needed since n = sgn e_D
   T gv[3][3], ge[3];
-  prim_grads<T>(uK, irK, vK, q, gv, ge);
-  const T mu = T(ph.mu), kap = T(ph.kappa);
+  prim_grads<T>(irK, b.E0 + wK[4], vK, q, gv, ge);
   const T div = gv[0][0] + gv[1][1] + gv[2][2];
   T tau[3];
 #pragma unroll
-  for (int i = 0; i < 3; ++i) tau[i] = mu * (gv[i][d] + gv[d][i]) - (i == d ? mu * T(2.0 / 3.0) * div : T(0));
+  for (int i = 0; i < 3; ++i) tau[i] = b.mu * (gv[i][D] + gv[D][i]) - (i == D ? b.mu * T(2.0 / 3.0) * div : T(0));
 #pragma unroll
   for (int i = 0; i < 3; ++i) out[1 + i] -= sgn * tau[i];
-  out[4] -= sgn * (vK[0] * tau[0] + vK[1] * tau[1] + vK[2] * tau[2] + kap * ge[d]);
+  out[4] -= sgn * (vK[0] * tau[0] + vK[1] * tau[1] + vK[2] * tau[2] + b.kappa * ge[D]);
 }
 
 // ---------------------------------------------------------------------
-// Block-wide deterministic sum of one double per thread -> thread 0.
+// Block-wide deterministic sum of one double per thread -> every thread.
 // ---------------------------------------------------------------------
 __device__ inline double block_sum(double v, double *sred) {
   sred[threadIdx.x] = v;
@@ -205,25 +233,28 @@ __device__ inline double block_sum(double v, double *sred) {
 // ---------------------------------------------------------------------
 // Kernel A: auxiliary gradient (solver.cpp:490-508, restated in strong
 // form: -M1^{-1} S1 + face lifts = Dn + M1^{-1} e_0 e_0^T, exactly).
-// One thread per node computes all 15 components and stores them straight
-// to HBM (consecutive threads -> consecutive nodes, coalesced).
+// Computed in FP64 and stored as TQ.  One thread per node computes all 15
+// components and stores them straight to HBM (consecutive threads ->
+// consecutive nodes, coalesced).
 // ---------------------------------------------------------------------
-template <int N, int E>
-__global__ void __launch_bounds__(NT) ldg_grad_kernel(const double *__restrict__ U, double *__restrict__ Q,
-                                                        MeshArgs m, Ops<N, double> op) {
+template <int N, int E, typename TQ>
+__global__ void __launch_bounds__(NT) ldg_grad_kernel(const double *__restrict__ U, TQ *__restrict__ Q,
+                                                        MeshArgs m, const Ops<N, double> *__restrict__ opp) {
   constexpr int NN = N * N * N, NF = N * N;
   __shared__ double su[E][NSTATE][NN];
   __shared__ double sj[E][3][NSTATE][NF];
   __shared__ double sD[N][N + 1];
+  __shared__ long snb[E][6];
   const int lb = logical_block(m);
   if (lb < 0) return;
   const long k0 = (long)lb * E;
   const int ne = (int)((m.nelem - k0) < E ? (m.nelem - k0) : E);
   const int tid = threadIdx.x;
+  fill_neighbors<E>(m, k0, ne, snb, tid);
 
   if (tid < N * (N + 1)) {
     const int i = tid / (N + 1), j = tid % (N + 1);
-    sD[i][j] = j < N ? op.Dn[i][j] : op.L0[i];
+    sD[i][j] = j < N ? opp->Dn[i][j] : opp->L0[i];
   }
   const double *Ub = U + k0 * NSTATE * NN;
   for (int w = tid; w < ne * NSTATE * NN; w += NT) (&su[0][0][0])[w] = Ub[w];
@@ -238,7 +269,7 @@ __global__ void __launch_bounds__(NT) ldg_grad_kernel(const double *__restrict__
     const int s = (r / NF) % NSTATE;
     const int ab = r % NF;
     const int a = ab % N, b = ab / N;
-    const long Kn = neighbor(m, k0 + e, d, -1);
+    const long Kn = snb[e][2 * d];
     const int base = line_base<N>(d, a, b), st = line_stride<N>(d);
     sj[e][d][s][ab] = su[e][s][base] - U[(Kn * NSTATE + s) * NN + base + (N - 1) * st];
   }
@@ -250,7 +281,7 @@ __global__ void __launch_bounds__(NT) ldg_grad_kernel(const double *__restrict__
     double rx[N + 1], ry[N + 1], rz[N + 1];
 #pragma unroll
     for (int j = 0; j <= N; ++j) { rx[j] = sD[ix][j]; ry[j] = sD[iy][j]; rz[j] = sD[iz][j]; }
-    double *Qe = Q + (k0 + e) * NGRAD * NN + node;
+    TQ *Qe = Q + (k0 + e) * NGRAD * NN + node;
 #pragma unroll
     for (int s = 0; s < NSTATE; ++s) {
       const double *us = su[e][s];
@@ -263,9 +294,9 @@ __global__ void __launch_bounds__(NT) ldg_grad_kernel(const double *__restrict__
         qy += ry[j] * us[ix + N * j + NF * iz];
         qz += rz[j] * us[ix + N * iy + NF * j];
       }
-      Qe[(0 * NSTATE + s) * NN] = qx * m.ialpha[0];
-      Qe[(1 * NSTATE + s) * NN] = qy * m.ialpha[1];
-      Qe[(2 * NSTATE + s) * NN] = qz * m.ialpha[2];
+      Qe[(0 * NSTATE + s) * NN] = TQ(qx * m.ialpha[0]);
+      Qe[(1 * NSTATE + s) * NN] = TQ(qy * m.ialpha[1]);
+      Qe[(2 * NSTATE + s) * NN] = TQ(qz * m.ialpha[2]);
     }
   }
 }
@@ -273,227 +304,359 @@ __global__ void __launch_bounds__(NT) ldg_grad_kernel(const double *__restrict__
 // ---------------------------------------------------------------------
 // Kernel B: face fluxes + volume integrals + inverse mass + RK epilogue.
 //
-// LDS per element: a 20-field volume buffer (which first serves as the
-// face buffer of one direction: 2 sides x 25 fields x N^2 points) and the
-// face residual (5 x N^3).
+// LDS per element: a 20-field volume buffer of type T (which first serves
+// as the face buffer of one direction: 2 sides x 25 fields x N^2 points),
+// the face residual (5 x N^3, type T), and for T = float an FP64 buffer for
+// the inverse-mass sweeps (for T = double those reuse the volume buffer).
 // ---------------------------------------------------------------------
-template <int N>
+template <int N, typename T>
 struct RhsLds {
   static constexpr int NN = N * N * N, NF = N * N;
   static constexpr int NV = NSTATE + NGRAD;          // 20 volume fields
   static constexpr int NFF = 25;                     // face fields: uK(5), uN(5), q~(15)
   static constexpr int SV = NV * NN;
   static constexpr int FB = 2 * NFF * NF;
-  static constexpr int BUF = SV > FB ? SV : FB;      // per element
+  static constexpr int BUF = SV > FB ? SV : FB;      // per element, even for every N
+  static constexpr bool SEP = sizeof(T) != sizeof(double);   // separate FP64 buffer
+  // occupancy target (waves per SIMD) handed to the register allocator
+  static constexpr int WAVES = SEP ? (N <= 5 ? 4 : 2) : (N <= 2 ? 4 : N <= 5 ? 3 : 2);
 };
 
-template <int N, int E>
-__global__ void __launch_bounds__(NT) ldg_rhs_kernel(const double *__restrict__ U, const double *__restrict__ Q,
-                                                       MeshArgs m, Ops<N, double> op, PhysArgs ph, StageArgs st) {
-  using L = RhsLds<N>;
-  constexpr int NN = L::NN, NF = L::NF, NV = L::NV, NFF = L::NFF, BUF = L::BUF;
-  __shared__ double sbuf[E][BUF];
-  __shared__ double sr[E][NSTATE][NN];
-  __shared__ double sred[NT];
-  const int lb = logical_block(m);
-  if (lb < 0) return;
-  const long k0 = (long)lb * E;
-  const int ne = (int)((m.nelem - k0) < E ? (m.nelem - k0) : E);
-  const int tid = threadIdx.x;
+// Operators in LDS: the compute-type copies used by the sweeps and the FP64
+// inverse mass.  Each phase loads the matrix it needs into a register array
+// before its loop (LDS loads produce VGPRs: the coefficients neither occupy
+// SGPRs nor spill).
+template <int N, typename T>
+struct SOps {
+  T I[N][N], D[N][N], w[N];
+  double Minv[N][N], wd[N];
+};
 
-  for (int w = tid; w < ne * NSTATE * NN; w += NT) (&sr[0][0][0])[w] = 0.0;
+template <int N, typename T>
+__device__ inline void load_sops(SOps<N, T> *so, const Ops<N, double> *op, int tid) {
+  for (int t = tid; t < N * N; t += NT) {
+    const int a = t / N, b = t % N;
+    so->I[a][b] = T(op->I[a][b]);
+    so->D[a][b] = T(op->D[a][b]);
+    so->Minv[a][b] = op->Minv[a][b];
+  }
+  if (tid < N) { so->w[tid] = T(op->w[tid]); so->wd[tid] = op->w[tid]; }
+}
 
-  // ================= faces, one direction at a time =================
-  for (int d = 0; d < 3; ++d) {
-    const int st_d = line_stride<N>(d);
-    const int ast = d == 0 ? N : 1;          // memory stride of the face coordinate a
-    // gather nodal face data straight from HBM and interpolate along a:
-    // fb[side][f][qa + N b]; f = uK(0..4), uN(5..9), q~(10..24)
-    for (int w = tid; w < ne * 2 * NFF * N; w += NT) {
-      const int e = w / (2 * NFF * N);
-      const int r = w - e * (2 * NFF * N);
-      const int side = r / (NFF * N);
-      const int f = (r / N) % NFF;
-      const int b = r % N;
-      const long K = k0 + e;
-      const int own = side ? (N - 1) * st_d : 0;
-      const int opp = side ? 0 : (N - 1) * st_d;
-      const double *src;
-      if (f < NSTATE) {
-        src = U + (K * NSTATE + f) * NN + own;
-      } else if (f < 2 * NSTATE) {
-        src = U + (neighbor(m, K, d, side ? 1 : -1) * NSTATE + (f - NSTATE)) * NN + opp;
-      } else if (side == 0) {
-        src = Q + (K * NGRAD + (f - 2 * NSTATE)) * NN;                        // own, i_d = 0
-      } else {
-        src = Q + (neighbor(m, K, d, 1) * NGRAD + (f - 2 * NSTATE)) * NN;     // neighbour's i_d = 0
-      }
-      src += line_base<N>(d, 0, b);
-      double v[N];
+template <int N, typename T>
+struct Mat {
+  T v[N][N];
+  __device__ inline void load(const T (&src)[N][N]) {
 #pragma unroll
-      for (int a = 0; a < N; ++a) v[a] = src[a * ast];
-      double *dst = &sbuf[e][(side * NFF + f) * NF + N * b];
+    for (int a = 0; a < N; ++a)
 #pragma unroll
-      for (int q = 0; q < N; ++q) {
-        double acc = 0.0;
+      for (int b = 0; b < N; ++b) v[a][b] = src[a][b];
+  }
+};
+
+// Face terms of one direction D (both faces of every element in the block):
+// gather nodal face data from HBM (own and neighbour traces, and the
+// gradient trace chosen by the one-sided switch), interpolate to the face
+// Gauss points, evaluate Roe + viscous fluxes, lift them back to the face
+// nodes and subtract them from the residual sr.
+template <int N, int E, typename T, int D>
+__device__ inline void rhs_face_dir(const double *__restrict__ U, const T *__restrict__ Q, const MeshArgs &m,
+                                    const SOps<N, T> *__restrict__ so, const Bg<T> &bg,
+                                    T (*__restrict__ sbuf)[RhsLds<N, T>::BUF], T (*__restrict__ sr)[NSTATE][N * N * N],
+                                    const long (*nb)[6], long k0, int ne, int tid) {
+  constexpr int NN = N * N * N, NF = N * N, NFF = RhsLds<N, T>::NFF;
+  constexpr int st_d = D == 0 ? 1 : D == 1 ? N : N * N;
+  constexpr int ast = D == 0 ? N : 1;          // memory stride of the face coordinate a
+  // gather nodal face data straight from HBM (all loads first) and
+  // interpolate along a: fb[side][f][qa + N b]; f = uK(0..4), uN(5..9), q~(10..24)
+  {
+    constexpr int GL = (E * 2 * NFF * N + NT - 1) / NT;
+    const int tot = ne * 2 * NFF * N;
+    T v[GL][N];
 #pragma unroll
-        for (int a = 0; a < N; ++a) acc += op.I[q][a] * v[a];
-        dst[q] = acc;
+    for (int it = 0; it < GL; ++it) {
+      const int w = tid + it * NT;
+      if (w < tot) {
+        const int e = w / (2 * NFF * N);
+        const int r = w - e * (2 * NFF * N);
+        const int side = r / (NFF * N);
+        const int f = (r / N) % NFF;
+        const int b = r % N;
+        const long K = k0 + e;
+        const int own = side ? (N - 1) * st_d : 0;
+        const int opp = side ? 0 : (N - 1) * st_d;
+        const int lb = line_base<N>(D, 0, b);
+        if (f < 2 * NSTATE) {
+          const int s = f < NSTATE ? f : f - NSTATE;
+          const double *src = f < NSTATE ? U + (K * NSTATE + s) * NN + own + lb
+                                         : U + (nb[e][2 * D + side] * NSTATE + s) * NN + opp + lb;
+          const double bgs = s == 0 ? (double)bg.rho0 : s == 4 ? (double)bg.E0 : 0.0;
+#pragma unroll
+          for (int a = 0; a < N; ++a) v[it][a] = T(src[a * ast] - bgs);
+        } else {
+          const long Ke = side == 0 ? K : nb[e][2 * D + 1];     // own / neighbour's i_d = 0 slice
+          const T *src = Q + (Ke * NGRAD + (f - 2 * NSTATE)) * NN + lb;
+#pragma unroll
+          for (int a = 0; a < N; ++a) v[it][a] = src[a * ast];
+        }
       }
     }
-    __syncthreads();
-    // interpolate along b (in place)
+    Mat<N, T> I;
+    I.load(so->I);
+#pragma unroll
+    for (int it = 0; it < GL; ++it) {
+      const int w = tid + it * NT;
+      if (w < tot) {
+        const int e = w / (2 * NFF * N);
+        const int r = w - e * (2 * NFF * N);
+        T *dst = &sbuf[e][(r / N) * NF + N * (r % N)];
+#pragma unroll
+        for (int q = 0; q < N; ++q) {
+          T acc = T(0);
+#pragma unroll
+          for (int a = 0; a < N; ++a) acc += I.v[q][a] * v[it][a];
+          dst[q] = acc;
+        }
+      }
+    }
+  }
+  __syncthreads();
+  // interpolate along b (in place)
+  {
+    Mat<N, T> I;
+    I.load(so->I);
     for (int w = tid; w < ne * 2 * NFF * N; w += NT) {
       const int e = w / (2 * NFF * N);
       const int r = w - e * (2 * NFF * N);
-      const int row = r / N;                  // side * NFF + f
-      const int qa = r % N;
-      double *p = &sbuf[e][row * NF + qa];
-      double v[N];
+      T *p = &sbuf[e][(r / N) * NF + r % N];
+      T v[N];
 #pragma unroll
       for (int b = 0; b < N; ++b) v[b] = p[b * N];
 #pragma unroll
       for (int q = 0; q < N; ++q) {
-        double acc = 0.0;
+        T acc = T(0);
 #pragma unroll
-        for (int b = 0; b < N; ++b) acc += op.I[q][b] * v[b];
+        for (int b = 0; b < N; ++b) acc += I.v[q][b] * v[b];
         p[q * N] = acc;
       }
     }
-    __syncthreads();
-    // flux at each face Gauss point, weighted for the lift
-    for (int w = tid; w < ne * 2 * NF; w += NT) {
-      const int e = w / (2 * NF);
-      const int r = w - e * (2 * NF);
-      const int side = r / NF;
-      const int ab = r % NF;
-      const double *fb = &sbuf[e][side * NFF * NF + ab];
-      double uK[5], uN[5], qq[15], out[5];
+  }
+  __syncthreads();
+  // flux at each face Gauss point, weighted for the lift
+  for (int w = tid; w < ne * 2 * NF; w += NT) {
+    const int e = w / (2 * NF);
+    const int r = w - e * (2 * NF);
+    const int side = r / NF;
+    const int ab = r % NF;
+    T *fb = &sbuf[e][side * NFF * NF + ab];
+    T uK[5], uN[5], qq[15], out[5];
 #pragma unroll
-      for (int s = 0; s < 5; ++s) { uK[s] = fb[s * NF]; uN[s] = fb[(5 + s) * NF]; }
+    for (int s = 0; s < 5; ++s) { uK[s] = fb[s * NF]; uN[s] = fb[(5 + s) * NF]; }
 #pragma unroll
-      for (int k = 0; k < 15; ++k) qq[k] = fb[(10 + k) * NF];
-      face_flux<double>(uK, uN, qq, d, side ? 1.0 : -1.0, ph, out);
-      const double wt = op.w[ab % N] * op.w[ab / N] * m.ialpha[d];
-      double *o = &sbuf[e][side * NFF * NF + ab];
+    for (int k = 0; k < 15; ++k) qq[k] = fb[(10 + k) * NF];
+    face_flux<T, D>(uK, uN, qq, side ? T(1) : T(-1), bg, out);
+    const T wt = so->w[ab % N] * so->w[ab / N] * T(m.ialpha[D]);
 #pragma unroll
-      for (int s = 0; s < 5; ++s) o[s * NF] = out[s] * wt;
-    }
-    __syncthreads();
-    // lift: I^T along a (in place)
+    for (int s = 0; s < 5; ++s) fb[s * NF] = out[s] * wt;
+  }
+  __syncthreads();
+  // lift: I^T along a (in place)
+  {
+    Mat<N, T> I;
+    I.load(so->I);
     for (int w = tid; w < ne * 2 * NSTATE * N; w += NT) {
       const int e = w / (2 * NSTATE * N);
       const int r = w - e * (2 * NSTATE * N);
       const int side = r / (NSTATE * N);
       const int s = (r / N) % NSTATE;
       const int qb = r % N;
-      double *p = &sbuf[e][(side * NFF + s) * NF + N * qb];
-      double v[N];
+      T *p = &sbuf[e][(side * NFF + s) * NF + N * qb];
+      T v[N];
 #pragma unroll
       for (int q = 0; q < N; ++q) v[q] = p[q];
 #pragma unroll
       for (int i = 0; i < N; ++i) {
-        double acc = 0.0;
+        T acc = T(0);
 #pragma unroll
-        for (int q = 0; q < N; ++q) acc += op.I[q][i] * v[q];
+        for (int q = 0; q < N; ++q) acc += I.v[q][i] * v[q];
         p[i] = acc;
       }
     }
-    __syncthreads();
-    // I^T along b, subtracted into the face nodes of the residual
+  }
+  __syncthreads();
+  // I^T along b, subtracted into the face nodes of the residual
+  {
+    Mat<N, T> I;
+    I.load(so->I);
     for (int w = tid; w < ne * 2 * NSTATE * N; w += NT) {
       const int e = w / (2 * NSTATE * N);
       const int r = w - e * (2 * NSTATE * N);
       const int side = r / (NSTATE * N);
       const int s = (r / N) % NSTATE;
       const int a = r % N;
-      const double *p = &sbuf[e][(side * NFF + s) * NF + a];
-      double v[N];
+      const T *p = &sbuf[e][(side * NFF + s) * NF + a];
+      T v[N];
 #pragma unroll
       for (int q = 0; q < N; ++q) v[q] = p[q * N];
       const int off = side ? (N - 1) * st_d : 0;
 #pragma unroll
       for (int b = 0; b < N; ++b) {
-        double acc = 0.0;
+        T acc = T(0);
 #pragma unroll
-        for (int q = 0; q < N; ++q) acc += op.I[q][b] * v[q];
-        sr[e][s][line_base<N>(d, a, b) + off] -= acc;
+        for (int q = 0; q < N; ++q) acc += I.v[q][b] * v[q];
+        sr[e][s][line_base<N>(D, a, b) + off] -= acc;
       }
     }
-    __syncthreads();
   }
+  __syncthreads();
+}
+
+template <int N, int E, typename T, bool SEP>
+struct Fp64Buf;
+template <int N, int E, typename T>
+struct Fp64Buf<N, E, T, true> { double v[E][NSTATE * N * N * N]; };
+template <int N, int E, typename T>
+struct Fp64Buf<N, E, T, false> { double v[1][1]; };
+
+template <int N, int E, typename T>
+__global__ void __launch_bounds__(NT, (RhsLds<N, T>::WAVES))
+ldg_rhs_kernel(const double *__restrict__ U, const T *__restrict__ Q, MeshArgs m,
+               const Ops<N, double> *__restrict__ opp, PhysArgs ph, StageArgs st) {
+  using L = RhsLds<N, T>;
+  constexpr int NN = L::NN, NF = L::NF, NV = L::NV, BUF = L::BUF;
+  __shared__ __align__(16) T sbuf[E][BUF];
+  __shared__ T sr[E][NSTATE][NN];
+  __shared__ Fp64Buf<N, E, T, L::SEP> smx;
+  __shared__ double sred[NT];
+  __shared__ SOps<N, T> sops;
+  __shared__ long snb[E][6];
+  const int lb = logical_block(m);
+  if (lb < 0) return;
+  const long k0 = (long)lb * E;
+  const int ne = (int)((m.nelem - k0) < E ? (m.nelem - k0) : E);
+  const int tid = threadIdx.x;
+  const SOps<N, T> *so = &sops;
+  const Bg<T> bg(ph);
+  // FP64 view used by the inverse-mass sweeps and the epilogue
+  auto md = [&](int e) -> double * {
+    if constexpr (L::SEP) return smx.v[e];
+    else return reinterpret_cast<double *>(&sbuf[e][0]);
+  };
+
+  fill_neighbors<E>(m, k0, ne, snb, tid);
+  load_sops<N, T>(&sops, opp, tid);
+  for (int w = tid; w < ne * NSTATE * NN; w += NT) (&sr[0][0][0])[w] = T(0);
+  __syncthreads();
+
+  // ================= faces, one direction at a time =================
+  rhs_face_dir<N, E, T, 0>(U, Q, m, so, bg, sbuf, sr, snb, k0, ne, tid);
+  rhs_face_dir<N, E, T, 1>(U, Q, m, so, bg, sbuf, sr, snb, k0, ne, tid);
+  rhs_face_dir<N, E, T, 2>(U, Q, m, so, bg, sbuf, sr, snb, k0, ne, tid);
 
   // ================= volume =================
-  // load U and Q x-lines from HBM and interpolate along x
-  for (int w = tid; w < ne * NV * NF; w += NT) {
-    const int e = w / (NV * NF);
-    const int r = w - e * (NV * NF);
-    const int f = r / NF;
-    const int yz = r % NF;
-    const long K = k0 + e;
-    const double *src = f < NSTATE ? U + (K * NSTATE + f) * NN + N * yz
-                                   : Q + (K * NGRAD + (f - NSTATE)) * NN + N * yz;
-    double v[N];
+  // load U (minus the background) and Q x-lines from HBM, all loads of the
+  // thread first, and interpolate along x
+  {
+    constexpr int VL = (E * NV * NF + NT - 1) / NT;
+    const int tot = ne * NV * NF;
+    T v[VL][N];
 #pragma unroll
-    for (int i = 0; i < N; ++i) v[i] = src[i];
-    double *dst = &sbuf[e][f * NN + N * yz];
+    for (int it = 0; it < VL; ++it) {
+      const int w = tid + it * NT;
+      if (w < tot) {
+        const int e = w / (NV * NF);
+        const int r = w - e * (NV * NF);
+        const int f = r / NF;
+        const int yz = r % NF;
+        const long K = k0 + e;
+        if (f < NSTATE) {
+          const double *src = U + (K * NSTATE + f) * NN + N * yz;
+          const double bgs = f == 0 ? ph.rho0 : f == 4 ? ph.E0 : 0.0;
 #pragma unroll
-    for (int q = 0; q < N; ++q) {
-      double acc = 0.0;
+          for (int i = 0; i < N; ++i) v[it][i] = T(src[i] - bgs);
+        } else {
+          const T *src = Q + (K * NGRAD + (f - NSTATE)) * NN + N * yz;
 #pragma unroll
-      for (int i = 0; i < N; ++i) acc += op.I[q][i] * v[i];
-      dst[q] = acc;
+          for (int i = 0; i < N; ++i) v[it][i] = src[i];
+        }
+      }
+    }
+    Mat<N, T> I;
+    I.load(so->I);
+#pragma unroll
+    for (int it = 0; it < VL; ++it) {
+      const int w = tid + it * NT;
+      if (w < tot) {
+        const int e = w / (NV * NF);
+        const int r = w - e * (NV * NF);
+        T *dst = &sbuf[e][(r / NF) * NN + N * (r % NF)];
+#pragma unroll
+        for (int q = 0; q < N; ++q) {
+          T acc = T(0);
+#pragma unroll
+          for (int i = 0; i < N; ++i) acc += I.v[q][i] * v[it][i];
+          dst[q] = acc;
+        }
+      }
     }
   }
   __syncthreads();
   // y then z sweeps (in place)
 #pragma unroll
   for (int dd = 1; dd < 3; ++dd) {
-    for (int w = tid; w < ne * NV * NF; w += NT) {
-      const int e = w / (NV * NF);
-      const int r = w - e * (NV * NF);
-      const int f = r / NF;
-      const int ab = r % NF;
-      const int ls = line_stride<N>(dd);
-      double *p = &sbuf[e][f * NN + line_base<N>(dd, ab % N, ab / N)];
-      double v[N];
+    {
+      Mat<N, T> I;
+      I.load(so->I);
+      for (int w = tid; w < ne * NV * NF; w += NT) {
+        const int e = w / (NV * NF);
+        const int r = w - e * (NV * NF);
+        const int f = r / NF;
+        const int ab = r % NF;
+        const int ls = line_stride<N>(dd);
+        T *p = &sbuf[e][f * NN + line_base<N>(dd, ab % N, ab / N)];
+        T v[N];
 #pragma unroll
-      for (int i = 0; i < N; ++i) v[i] = p[i * ls];
+        for (int i = 0; i < N; ++i) v[i] = p[i * ls];
 #pragma unroll
-      for (int q = 0; q < N; ++q) {
-        double acc = 0.0;
+        for (int q = 0; q < N; ++q) {
+          T acc = T(0);
 #pragma unroll
-        for (int i = 0; i < N; ++i) acc += op.I[q][i] * v[i];
-        p[q * ls] = acc;
+          for (int i = 0; i < N; ++i) acc += I.v[q][i] * v[i];
+          p[q * ls] = acc;
+        }
       }
     }
     __syncthreads();
   }
 
-  // pointwise volume flux, weighted; optional Gauss diagnostics
+  // pointwise volume flux, weighted; optional Gauss diagnostics (FP64 sums)
   double dke = 0.0, dens = 0.0;
   for (int w = tid; w < ne * NN; w += NT) {
     const int e = w / NN, q = w - e * NN;
-    double *pv = &sbuf[e][q];
-    double u[5], g[15], F[15];
+    T *pv = &sbuf[e][q];
+    T u[5], g[15], F[15];
 #pragma unroll
     for (int s = 0; s < 5; ++s) u[s] = pv[s * NN];
 #pragma unroll
     for (int k = 0; k < 15; ++k) g[k] = pv[(NSTATE + k) * NN];
-    const double wq = op.w[q % N] * op.w[(q / N) % N] * op.w[q / NF];
     if (st.diag_part) {
-      const double ir = 1.0 / u[0];
-      const double v[3] = {u[1] * ir, u[2] * ir, u[3] * ir};
-      double gv[3][3], ge[3];
-      prim_grads<double>(u, ir, v, g, gv, ge);
+      const double rho = ph.rho0 + (double)u[0];
+      const double ir = 1.0 / rho;
+      const double v[3] = {(double)u[1] * ir, (double)u[2] * ir, (double)u[3] * ir};
+      double gd[15], gv[3][3], ge[3];
+#pragma unroll
+      for (int k = 0; k < 15; ++k) gd[k] = g[k];
+      prim_grads<double>(ir, ph.E0 + (double)u[4], v, gd, gv, ge);
       const double w1 = gv[2][1] - gv[1][2], w2 = gv[0][2] - gv[2][0], w3 = gv[1][0] - gv[0][1];
-      dke += wq * 0.5 * (u[1] * v[0] + u[2] * v[1] + u[3] * v[2]);
-      dens += wq * 0.5 * u[0] * (w1 * w1 + w2 * w2 + w3 * w3);
+      const double wq = so->wd[q % N] * so->wd[(q / N) % N] * so->wd[q / NF];
+      dke += wq * 0.5 * ((double)u[1] * v[0] + (double)u[2] * v[1] + (double)u[3] * v[2]);
+      dens += wq * 0.5 * rho * (w1 * w1 + w2 * w2 + w3 * w3);
     }
-    volume_flux<double>(u, g, ph, F);
+    volume_flux<T>(u, g, bg, F);
+    const T wq = so->w[q % N] * so->w[(q / N) % N] * so->w[q / NF];
 #pragma unroll
     for (int l = 0; l < 3; ++l) {
-      const double sc = wq * m.ialpha[l];
+      const T sc = wq * T(m.ialpha[l]);
 #pragma unroll
       for (int s = 0; s < 5; ++s) pv[(NSTATE + 5 * l + s) * NN] = F[s + 5 * l] * sc;
     }
@@ -507,128 +670,177 @@ __global__ void __launch_bounds__(NT) ldg_rhs_kernel(const double *__restrict__ 
   __syncthreads();
 
   // test-function contraction: z sweep (I^T on F_x, F_y; D^T on F_z)
-  for (int w = tid; w < ne * NGRAD * NF; w += NT) {
-    const int e = w / (NGRAD * NF);
-    const int r = w - e * (NGRAD * NF);
-    const int f = r / NF;            // 0..14 = s + 5 l
-    const int ab = r % NF;
-    double *p = &sbuf[e][(NSTATE + f) * NN + ab];
-    double v[N];
+  {
+    Mat<N, T> I;
+    I.load(so->I);
+    for (int w = tid; w < ne * 10 * NF; w += NT) {
+      const int e = w / (10 * NF);
+      const int r = w - e * (10 * NF);
+      T *p = &sbuf[e][NSTATE * NN + (r / NF) * NN + r % NF];
+      T v[N];
 #pragma unroll
-    for (int q = 0; q < N; ++q) v[q] = p[q * NF];
-    if (f >= 10) {
+      for (int q = 0; q < N; ++q) v[q] = p[q * NF];
 #pragma unroll
       for (int i = 0; i < N; ++i) {
-        double acc = 0.0;
+        T acc = T(0);
 #pragma unroll
-        for (int q = 0; q < N; ++q) acc += op.D[q][i] * v[q];
+        for (int q = 0; q < N; ++q) acc += I.v[q][i] * v[q];
         p[i * NF] = acc;
       }
-    } else {
+    }
+  }
+  {
+    Mat<N, T> Dm;
+    Dm.load(so->D);
+    for (int w = tid; w < ne * NSTATE * NF; w += NT) {
+      const int e = w / (NSTATE * NF);
+      const int r = w - e * (NSTATE * NF);
+      T *p = &sbuf[e][(NSTATE + 10) * NN + (r / NF) * NN + r % NF];
+      T v[N];
+#pragma unroll
+      for (int q = 0; q < N; ++q) v[q] = p[q * NF];
 #pragma unroll
       for (int i = 0; i < N; ++i) {
-        double acc = 0.0;
+        T acc = T(0);
 #pragma unroll
-        for (int q = 0; q < N; ++q) acc += op.I[q][i] * v[q];
+        for (int q = 0; q < N; ++q) acc += Dm.v[q][i] * v[q];
         p[i * NF] = acc;
       }
     }
   }
   __syncthreads();
   // y sweep: F_x <- I^T F_x ; G <- D^T F_y + I^T F_z (stored in the F_y slot)
-  for (int w = tid; w < ne * NSTATE * NF; w += NT) {
-    const int e = w / (NSTATE * NF);
-    const int r = w - e * (NSTATE * NF);
-    const int s = r / NF;
-    const int ab = r % NF;
-    const int base = (ab % N) + NF * (ab / N);
-    double *px = &sbuf[e][(NSTATE + s) * NN + base];
-    double *py = &sbuf[e][(NSTATE + 5 + s) * NN + base];
-    const double *pz = &sbuf[e][(NSTATE + 10 + s) * NN + base];
-    double vx[N], vy[N], vz[N];
-#pragma unroll
-    for (int q = 0; q < N; ++q) { vx[q] = px[q * N]; vy[q] = py[q * N]; vz[q] = pz[q * N]; }
-#pragma unroll
-    for (int i = 0; i < N; ++i) {
-      double ax = 0.0, ag = 0.0;
-#pragma unroll
-      for (int q = 0; q < N; ++q) {
-        ax += op.I[q][i] * vx[q];
-        ag += op.D[q][i] * vy[q] + op.I[q][i] * vz[q];
-      }
-      px[i * N] = ax;
-      py[i * N] = ag;
-    }
-  }
-  __syncthreads();
-  // x sweep: r = D^T F_x + I^T G + face residual, then M1^{-1} along x
-  for (int w = tid; w < ne * NSTATE * NF; w += NT) {
-    const int e = w / (NSTATE * NF);
-    const int r = w - e * (NSTATE * NF);
-    const int s = r / NF;
-    const int ab = r % NF;
-    const int base = N * ab;
-    const double *px = &sbuf[e][(NSTATE + s) * NN + base];
-    const double *pg = &sbuf[e][(NSTATE + 5 + s) * NN + base];
-    double vx[N], vg[N], rr[N];
-#pragma unroll
-    for (int q = 0; q < N; ++q) { vx[q] = px[q]; vg[q] = pg[q]; }
-#pragma unroll
-    for (int i = 0; i < N; ++i) {
-      double acc = sr[e][s][base + i];
-#pragma unroll
-      for (int q = 0; q < N; ++q) acc += op.D[q][i] * vx[q] + op.I[q][i] * vg[q];
-      rr[i] = acc;
-    }
-    double *po = &sbuf[e][s * NN + base];
-#pragma unroll
-    for (int i = 0; i < N; ++i) {
-      double acc = 0.0;
-#pragma unroll
-      for (int j = 0; j < N; ++j) acc += op.Minv[i][j] * rr[j];
-      po[i] = acc;
-    }
-  }
-  __syncthreads();
-  // M1^{-1} along y, then along z
-#pragma unroll
-  for (int dd = 1; dd < 3; ++dd) {
+  {
+    Mat<N, T> I;
+    I.load(so->I);
     for (int w = tid; w < ne * NSTATE * NF; w += NT) {
       const int e = w / (NSTATE * NF);
       const int r = w - e * (NSTATE * NF);
       const int s = r / NF;
       const int ab = r % NF;
-      const int ls = line_stride<N>(dd);
-      double *p = &sbuf[e][s * NN + line_base<N>(dd, ab % N, ab / N)];
-      double v[N];
+      const int base = (ab % N) + NF * (ab / N);
+      T *px = &sbuf[e][(NSTATE + s) * NN + base];
+      T *py = &sbuf[e][(NSTATE + 5 + s) * NN + base];
+      const T *pz = &sbuf[e][(NSTATE + 10 + s) * NN + base];
+      T vx[N], vy[N], vz[N];
 #pragma unroll
-      for (int i = 0; i < N; ++i) v[i] = p[i * ls];
+      for (int q = 0; q < N; ++q) { vx[q] = px[q * N]; vy[q] = py[q * N]; vz[q] = pz[q * N]; }
+#pragma unroll
+      for (int i = 0; i < N; ++i) {
+        T ax = T(0), ag = T(0);
+#pragma unroll
+        for (int q = 0; q < N; ++q) {
+          ax += I.v[q][i] * vx[q];
+          ag += so->D[q][i] * vy[q] + I.v[q][i] * vz[q];
+        }
+        px[i * N] = ax;
+        py[i * N] = ag;
+      }
+    }
+  }
+  __syncthreads();
+  // x sweep: r = D^T F_x + I^T G + face residual (T), then M1^{-1} along x
+  // in FP64.  With T = double the result overwrites the state fields 0..4
+  // of the same element, which no thread reads any more at this point.
+  {
+    Mat<N, T> I;
+    I.load(so->I);
+    for (int w = tid; w < ne * NSTATE * NF; w += NT) {
+      const int e = w / (NSTATE * NF);
+      const int r = w - e * (NSTATE * NF);
+      const int s = r / NF;
+      const int ab = r % NF;
+      const int base = N * ab;
+      const T *px = &sbuf[e][(NSTATE + s) * NN + base];
+      const T *pg = &sbuf[e][(NSTATE + 5 + s) * NN + base];
+      T vx[N], vg[N];
+      double rr[N];
+#pragma unroll
+      for (int q = 0; q < N; ++q) { vx[q] = px[q]; vg[q] = pg[q]; }
+#pragma unroll
+      for (int i = 0; i < N; ++i) {
+        T acc = sr[e][s][base + i];
+#pragma unroll
+        for (int q = 0; q < N; ++q) acc += so->D[q][i] * vx[q] + I.v[q][i] * vg[q];
+        rr[i] = (double)acc;
+      }
+      double *po = md(e) + s * NN + base;
 #pragma unroll
       for (int i = 0; i < N; ++i) {
         double acc = 0.0;
 #pragma unroll
-        for (int j = 0; j < N; ++j) acc += op.Minv[i][j] * v[j];
-        p[i * ls] = acc;
+        for (int j = 0; j < N; ++j) acc += so->Minv[i][j] * rr[j];
+        po[i] = acc;
+      }
+    }
+  }
+  __syncthreads();
+  // M1^{-1} along y, then along z (FP64)
+#pragma unroll
+  for (int dd = 1; dd < 3; ++dd) {
+    {
+      Mat<N, double> Mi;
+      Mi.load(so->Minv);
+      for (int w = tid; w < ne * NSTATE * NF; w += NT) {
+        const int e = w / (NSTATE * NF);
+        const int r = w - e * (NSTATE * NF);
+        const int s = r / NF;
+        const int ab = r % NF;
+        const int ls = line_stride<N>(dd);
+        double *p = md(e) + s * NN + line_base<N>(dd, ab % N, ab / N);
+        double v[N];
+#pragma unroll
+        for (int i = 0; i < N; ++i) v[i] = p[i * ls];
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+          double acc = 0.0;
+#pragma unroll
+          for (int j = 0; j < N; ++j) acc += Mi.v[i][j] * v[j];
+          p[i * ls] = acc;
+        }
       }
     }
     __syncthreads();
   }
 
-  // ---- epilogue: k, RK4 accumulation and next stage input
-  const long gofs = k0 * NSTATE * NN;
-  for (int w = tid; w < ne * NSTATE * NN; w += NT) {
-    const int e = w / (NSTATE * NN), r = w - e * (NSTATE * NN);
-    const double k = sbuf[e][r];
-    const long gi = gofs + w;
-    if (st.kout) st.kout[gi] = k;
-    if (st.acc || st.unext) {
-      const double u0 = st.u0[gi];
-      if (st.acc) {
-        const double a = st.acc_init ? u0 + st.dtb * k : st.acc[gi] + st.dtb * k;
-        st.acc[gi] = a;
-        if (st.ke_part) sbuf[e][NSTATE * NN + r] = a;   // fields 5..9 are free again
+  // ---- epilogue (FP64): k, RK4 accumulation and next stage input.  All of
+  // a thread's HBM reads are issued before any is used.
+  {
+    constexpr int EPI = (E * NSTATE * NN + NT - 1) / NT;
+    const int tot = ne * NSTATE * NN;
+    const long gofs = k0 * NSTATE * NN;
+    const bool need_u0 = st.acc || st.unext;
+    const bool need_acc = st.acc && !st.acc_init;
+    double ru[EPI], ra[EPI], kv[EPI];
+#pragma unroll
+    for (int it = 0; it < EPI; ++it) {
+      const int w = tid + it * NT;
+      ru[it] = 0.0;
+      ra[it] = 0.0;
+      kv[it] = 0.0;
+      if (w < tot) {
+        if (need_u0) ru[it] = st.u0[gofs + w];
+        if (need_acc) ra[it] = st.acc[gofs + w];
+        const int e = w / (NSTATE * NN), r = w - e * (NSTATE * NN);
+        kv[it] = md(e)[r];
       }
-      if (st.unext) st.unext[gi] = u0 + st.dta * k;
+    }
+    if (st.ke_part) __syncthreads();   // every k read before the new state reuses the buffer
+#pragma unroll
+    for (int it = 0; it < EPI; ++it) {
+      const int w = tid + it * NT;
+      if (w < tot) {
+        const int e = w / (NSTATE * NN), r = w - e * (NSTATE * NN);
+        const double k = kv[it];
+        const long gi = gofs + w;
+        if (st.kout) st.kout[gi] = k;
+        if (st.acc) {
+          const double a = (st.acc_init ? ru[it] : ra[it]) + st.dtb * k;
+          st.acc[gi] = a;
+          if (st.ke_part) md(e)[r] = a;
+        }
+        if (st.unext) st.unext[gi] = ru[it] + st.dta * k;
+      }
     }
   }
   if (st.ke_part) {
@@ -636,16 +848,15 @@ __global__ void __launch_bounds__(NT) ldg_rhs_kernel(const double *__restrict__ 
     double kel = 0.0;
     for (int w = tid; w < ne * NN; w += NT) {
       const int e = w / NN, q = w - e * NN;
-      const double *a = &sbuf[e][NSTATE * NN];
+      const double *a = md(e);
       const double rho = a[q], m1 = a[NN + q], m2 = a[2 * NN + q], m3 = a[3 * NN + q];
-      const double wq = op.w[q % N] * op.w[(q / N) % N] * op.w[q / NF];
+      const double wq = so->wd[q % N] * so->wd[(q / N) % N] * so->wd[q / NF];
       kel += wq * (m1 * m1 + m2 * m2 + m3 * m3) / (2.0 * rho);
     }
     const double t = block_sum(kel, sred);
     if (tid == 0) st.ke_part[lb] = t;
   }
 }
-
 
 // Ordered sum of per-workgroup partials: out[j] = scale * sum_b part[b*stride + j].
 __global__ void __launch_bounds__(NT) ldg_finalize_kernel(const double *__restrict__ part, int nparts, int stride,
