@@ -1,19 +1,29 @@
 #!/usr/bin/env python3
 """Throughput of the LDG right-hand side + RK4 stage pipeline on one MI355X.
 
-A step is one RK4 step (4 stages, 8 kernel launches plus the kinetic-energy
+A step is one RK4 step (4 stages: 8 kernel launches plus the kinetic-energy
 reduction) of the compressible Taylor-Green vortex at 64^3 elements, p=4
 (163,840,000 DOF), with the state resident in HBM.  The metric is DOF-stage
 updates per second: 4 * steps * K * (p+1)^3 * 5 / wall.
 
-Prints one JSON line.  The CPU baseline is the in-repo FP64 oracle (dense
-element operators, OpenMP over elements) timed on a bounded sample.
+Both paths are timed on the same workload: the mixed path (FP32
+interpolation / flux / contractions, FP64 state, RK4, inverse mass and
+reductions) is `value`; the FP64 plain path is reported beside it with the
+speed-up, and the mixed state's relative L2 error against the FP64 state
+after the timed steps (both start from the same initial state) together with
+the error of the -dEk/dt series.  `--validate T` instead runs both paths to
+time T and prints the errors only.
+
+The CPU baseline is the in-repo FP64 oracle (dense element operators, OpenMP
+over elements) timed on a bounded sample.  Prints one JSON line.
 """
 import argparse
 import json
 import os
 import sys
 import time
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "local-dg_amd"))
@@ -41,6 +51,67 @@ def cpu_baseline(p, ne, steps):
                       f"{wall:.1f} s"}
 
 
+def alg_bytes(dof, mixed):
+    """Algorithmic HBM bytes per launch (8-byte state words; the gradient Q
+    is 3 fields per state variable, 4 bytes each when mixed):
+      grad: read U_s, write Q
+      rhs : read U_s, Q, u^n; acc read+write; write U_{s+1}; averaged over
+            the 4 stages (stage 1 reads U_s = u^n once and no acc, stage 4
+            writes no next input)."""
+    q = 3 * (4 if mixed else 8)
+    grad = (8 + q) * dof
+    rhs = ((8 + q + 8 + 8) + (8 + q + 8 + 16 + 8) * 2 + (8 + q + 8 + 16)) / 4 * dof
+    return grad, rhs
+
+
+def rel_l2(a, b):
+    return (np.sqrt(np.sum((a - b) ** 2, axis=(0, 2))) / np.sqrt(np.sum(b ** 2, axis=(0, 2)))).tolist()
+
+
+def dissipation(ke, dt):
+    ke = np.asarray(ke)
+    return -(ke[2:] - ke[:-2]) / (2 * dt)
+
+
+def timed_run(ldg_amd, p, nx, prec, steps, warmup):
+    g = ldg_amd.LDG(p, nx, precision=prec)
+    g.initial_condition()
+    dt = g.time_step(0.01)
+    ke_w = g.step(dt, warmup, record_ke=True)
+    t0 = time.perf_counter()
+    ke = g.step(dt, steps, record_ke=True)          # synchronises on return
+    wall = time.perf_counter() - t0
+    return g, dt, wall, np.concatenate([ke_w, ke])
+
+
+def ensemble_bench(ldg_amd, args):
+    """Seeded ensemble of uneven Taylor-Green instances (ldg_ensemble_sample):
+    mixed precision, tilings autotuned, timed with eager launches and with
+    graph replay over the same steps."""
+    inst = ldg_amd.sample_instances(args.seed, args.ensemble)
+    out = {"metric": "ensemble DOF-stage updates/s", "unit": "DOF-stage updates/s", "seed": args.seed,
+           "instances": len(inst), "steps": args.steps, "warmup": args.warmup}
+    ens = ldg_amd.Ensemble(inst, precision=ldg_amd.LDG_MIXED)
+    dof = ens.total_dof()
+    out["dof"] = dof
+    out["orders"] = {str(q): sum(1 for d in inst if d["p"] == q) for q in (2, 3, 4)}
+    ens.initial_condition()
+    out["autotune"] = ens.autotune()
+    dts = ens.time_steps(0.01)
+    for mode in ("eager", "graph"):
+        ens.set_graph(mode == "graph")
+        ens.step(dts, args.warmup + (args.warmup % 2))
+        t0 = time.perf_counter()
+        ke = ens.step(dts, args.steps)
+        wall = time.perf_counter() - t0
+        out[mode] = {"value": 4 * args.steps * dof / wall, "ms_per_step": wall / args.steps * 1e3}
+        assert np.all(np.isfinite(ke))
+    out["value"] = out["graph"]["value"]
+    out["graph_speedup"] = out["graph"]["value"] / out["eager"]["value"]
+    ens.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -48,10 +119,16 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--nx", type=int, default=64)
     ap.add_argument("--p", type=int, default=4)
-    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed",
+                    help="path reported as `value` (both are timed)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-nx", type=int, default=32)
     ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--validate", type=float, default=0.0, metavar="T",
+                    help="run both paths to time T and print their difference only")
+    ap.add_argument("--ensemble", type=int, default=0, metavar="N",
+                    help="instead: an N-instance seeded ensemble (mixed precision, autotuned, graph vs eager)")
+    ap.add_argument("--seed", type=int, default=0, help="ensemble sampler seed")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -73,31 +150,60 @@ def main():
         sys.exit(2)
 
     import ldg_amd
-    prec = ldg_amd.LDG_FP64 if args.precision == "fp64" else ldg_amd.LDG_MIXED
     p, nx = args.p, args.nx
-    g = ldg_amd.LDG(p, nx, precision=prec)
-    g.initial_condition()
-    dt = g.time_step(0.01)
-    dof = g.K * g.dofs * 5
 
-    g.step(dt, args.warmup, record_ke=True)
-    t0 = time.perf_counter()
-    ke = g.step(dt, args.steps, record_ke=True)      # synchronises on return
-    wall = time.perf_counter() - t0
-    value = 4 * args.steps * dof / wall
+    if args.ensemble:
+        print(json.dumps(ensemble_bench(ldg_amd, args)), flush=True)
+        return
 
-    # per-kernel device time (HIP events on the library's stream), 2 steps
-    kt = g.kernel_times(dt, 2)
-    # algorithmic HBM bytes per launch, FP64 words per DOF:
-    #   grad: read U_s, write Q (3)                                   = 4
-    #   rhs : read U_s, Q (3), u^n; acc read+write; write U_{s+1};
-    #         averaged over the 4 stages (stage 1 does not read acc,
-    #         stage 4 writes no next input): (6 + 8 + 8 + 7) / 4       = 7.25
-    bytes_grad = 4 * 8 * dof
-    bytes_rhs = 7.25 * 8 * dof
-    ach_rhs = bytes_rhs / (kt["rhs_ms"] * 1e-3) / 1e9
-    ach_grad = bytes_grad / (kt["grad_ms"] * 1e-3) / 1e9
-    diag = g.diagnostics()
+    if args.validate > 0:
+        out = {"validate_t": args.validate, "workload": f"Taylor-Green {nx}^3 elements p={p}"}
+        res = {}
+        for name, prec in (("fp64", ldg_amd.LDG_FP64), ("mixed", ldg_amd.LDG_MIXED)):
+            g = ldg_amd.LDG(p, nx, precision=prec)
+            g.initial_condition()
+            dt0 = g.time_step(0.01)
+            n = int(np.ceil(args.validate / dt0))
+            dt = args.validate / n
+            kes, done, t0 = [], 0, time.perf_counter()
+            while done < n:
+                c = min(500, n - done)
+                kes.append(g.step(dt, c, record_ke=True))
+                done += c
+                print(f"{name}: {done}/{n} steps", file=sys.stderr, flush=True)
+            res[name] = (g.get_state(), np.concatenate(kes), dt, time.perf_counter() - t0, g.diagnostics())
+            g.close()
+        u64, ke64, dt, w64, d64 = res["fp64"]
+        u32, ke32, _, w32, d32 = res["mixed"]
+        d_ref = dissipation(ke64, dt)
+        out.update({"steps": int(len(ke64)), "dt": dt, "rel_l2_mixed_vs_fp64": rel_l2(u32, u64),
+                    "ke_rel_err": float(np.max(np.abs(ke32 - ke64) / ke64)),
+                    "dEk_dt_err": float(np.max(np.abs(dissipation(ke32, dt) - d_ref)) / np.max(np.abs(d_ref))),
+                    "enstrophy_fp64": d64["enstrophy"], "enstrophy_mixed": d32["enstrophy"],
+                    "wall_s": {"fp64": w64, "mixed": w32}})
+        print(json.dumps(out), flush=True)
+        return
+
+    runs = {}
+    for name, prec in (("fp64", ldg_amd.LDG_FP64), ("mixed", ldg_amd.LDG_MIXED)):
+        g, dt, wall, ke = timed_run(ldg_amd, p, nx, prec, args.steps, args.warmup)
+        dof = g.K * g.dofs * 5
+        u = g.get_state()
+        kt = g.kernel_times(dt, 2)          # per-kernel device time, HIP events on the library's stream
+        runs[name] = dict(K=g.K, dt=dt, wall=wall, ke=ke, dof=dof, kt=kt, u=u)
+        g.close()
+
+    main_name = args.precision
+    r = runs[main_name]
+    dof = r["dof"]
+    value = 4 * args.steps * dof / r["wall"]
+    fp64_value = 4 * args.steps * dof / runs["fp64"]["wall"]
+    mixed = main_name == "mixed"
+    b_grad, b_rhs = alg_bytes(dof, mixed)
+    ach_rhs = b_rhs / (r["kt"]["rhs_ms"] * 1e-3) / 1e9
+    d_ref = dissipation(runs["fp64"]["ke"], r["dt"])
+    d_mix = dissipation(runs["mixed"]["ke"], r["dt"])
+    err = rel_l2(runs["mixed"]["u"], runs["fp64"]["u"])
 
     out = {
         "metric": METRIC,
@@ -106,22 +212,29 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": wall / args.steps * 1e3,
+        "ms_per_step": r["wall"] / args.steps * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.precision == "fp64" else "f64 state / f32 flux",
+        "dtype": "f32 flux/quadrature + f64 state, RK4, inverse mass, reductions" if mixed else "f64",
         "data": "synthetic: compressible Taylor-Green initial state (Re 1600, Mach 0.1), no files",
         "config": {"workload": f"LDG RHS + RK4, Taylor-Green {nx}^3 elements p={p} ({dof} DOF), "
-                               f"dt = 0.01 dx/(2p+1)", "elements": g.K, "p": p, "dof": dof,
-                   "precision": args.precision},
+                               f"dt = 0.01 dx/(2p+1)", "elements": r["K"], "p": p, "dof": dof,
+                   "precision": main_name},
+        "fp64_value": fp64_value,
+        "speedup_vs_fp64": value / fp64_value,
+        "error_vs_fp64": {"after_steps": args.warmup + args.steps, "rel_l2_per_variable": err,
+                          "dEk_dt_err": float(np.max(np.abs(d_mix - d_ref)) / np.max(np.abs(d_ref)))},
         "roofline": {"bound": "hbm", "achieved": ach_rhs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": ach_rhs / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "ldg_rhs_kernel", "avg_ms": kt["rhs_ms"]},
-        "kernels": {"ldg_grad_kernel": {"avg_ms": kt["grad_ms"], "alg_GBs": ach_grad},
-                    "ldg_rhs_kernel": {"avg_ms": kt["rhs_ms"], "alg_GBs": ach_rhs}},
-        "ke_last": float(ke[-1]) if len(ke) else None,
-        "diagnostics": diag,
+                     "kernel": "ldg_rhs_kernel", "alg_bytes_per_launch": b_rhs, "avg_ms": r["kt"]["rhs_ms"]},
+        "kernels": {
+            name: {"ldg_grad_kernel": {"avg_ms": v["kt"]["grad_ms"],
+                                       "alg_GBs": alg_bytes(dof, name == "mixed")[0] / (v["kt"]["grad_ms"] * 1e-3) / 1e9},
+                   "ldg_rhs_kernel": {"avg_ms": v["kt"]["rhs_ms"],
+                                      "alg_GBs": alg_bytes(dof, name == "mixed")[1] / (v["kt"]["rhs_ms"] * 1e-3) / 1e9},
+                   "ms_per_step": v["wall"] / args.steps * 1e3}
+            for name, v in runs.items()},
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(p, args.cpu_nx, args.cpu_steps)

@@ -108,6 +108,62 @@ int ldg_diagnostics(ldg_handle *h, double out[3]);
    time, solver.cpp:344-372). */
 int ldg_kernel_times(ldg_handle *h, double dt, int nsteps, double ms[2]);
 
+/* ------------------------------------------------------------------------
+ * Ensembles: many independent solver instances advanced by the same kernel
+ * launches.  No reference counterpart: the reference runs one case per
+ * process (driver.cpp:20-60).  Instances of one order share one launch of
+ * each kernel; the domain, Pr, gamma, rho0, R, L and precision come from the
+ * base configuration, and each instance has its own order, mesh, Reynolds
+ * and Mach numbers, velocity scale and a phase shift of the Taylor-Green
+ * field.  The single-instance calls above act on instance 0; ldg_rhs,
+ * ldg_step and ldg_kernel_times need a handle of one instance.
+ * ------------------------------------------------------------------------ */
+typedef struct ldg_instance {
+  int p;                  /* polynomial order, 1..5 */
+  int nx, ny, nz;         /* elements per direction, each >= 3 */
+  double Re, M0, V0;
+  double phase[3];        /* initial state = Taylor-Green evaluated at x + phase */
+} ldg_instance;
+
+/* n instances with parameters drawn from seed: meshes 8^3, 12^3, 16^3 or
+   24^3, p in 2..4, Re log-uniform in [400, 3200], M0 uniform in
+   [0.05, 0.30], V0 = 1 + 0.05 N(0,1), phases uniform in [0, 2 pi). */
+int ldg_ensemble_sample(unsigned long long seed, int n, ldg_instance *out);
+
+/* Handle of n instances (state zero until set or initialised). */
+int ldg_ensemble_create(const ldg_config *base, const ldg_instance *inst, int n, ldg_handle **out);
+
+int ldg_instance_count(const ldg_handle *h, int *n);
+int ldg_instance_sizes(const ldg_handle *h, int i, int *p, int *dofs, int *nelem);
+
+/* ldg_time_step's rule for instance i. */
+int ldg_instance_time_step(const ldg_handle *h, int i, double cfl, double *dt);
+
+/* State of instance i, layout as for ldg_set_state with that instance's
+   dofs and K. */
+int ldg_instance_set_state(ldg_handle *h, int i, const double *u);
+int ldg_instance_get_state(ldg_handle *h, int i, double *u);
+
+/* ldg_diagnostics for instance i. */
+int ldg_instance_diagnostics(ldg_handle *h, int i, double out[3]);
+
+/* nsteps RK4 steps of every instance, instance i with step dt[i].  If ke is
+   not NULL, ke[s * n + i] receives instance i's reference kinetic energy
+   after step s. */
+int ldg_ensemble_step(ldg_handle *h, const double *dt, int nsteps, double *ke);
+
+/* enable != 0: ldg_step / ldg_ensemble_step replay a HIP graph of two RK4
+   steps instead of launching the kernels one by one (an odd last step is
+   launched eagerly).  Results are identical. */
+int ldg_set_graph(ldg_handle *h, int enable);
+
+/* Times the candidate elements-per-workgroup tilings of each kernel on the
+   resident state and keeps the fastest per order.  The state is not
+   changed.  If chosen is not NULL it receives, per order group in order of
+   first appearance, (p, elements per workgroup of the gradient kernel, of
+   the right-hand-side kernel): 3 ints per group, at most 5 groups. */
+int ldg_autotune(ldg_handle *h, int *chosen);
+
 #ifdef __cplusplus
 }
 #endif

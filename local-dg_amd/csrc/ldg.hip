@@ -1,20 +1,29 @@
 // ldg.hip -- C-ABI implementation of libldg.so (see include/ldg.h).
 //
-// Owns the device state of one solver instance and drives the two-kernel
-// RK4 stage pipeline on one HIP stream:
+// A handle owns one or more solver instances.  Instances of the same order
+// form a group whose arrays are stored back to back, so one launch of each
+// kernel advances every instance of the group (a plain solver is a group of
+// one instance).  Per group, on the handle's HIP stream:
 //   u      u^n (the reference's Solver::u)
-//   acc    u^n + dt sum_s b_s k_s, becomes u^{n+1} after stage 4
+//   acc    u^n + dt sum_s b_s k_s, becomes u^{n+1} after stage 4 (pointer swap)
 //   ua/ub  stage inputs U_s (double-buffered: stage s+1 is written while
 //          neighbours of stage s are still being read)
 //   q      LDG auxiliary gradient (15 fields per element; FP64, or FP32 in
 //          the mixed mode)
+// Steps can be captured into a HIP graph: two steps per graph, so that the
+// u/acc swap returns the arrays to their starting roles.  An odd number of
+// eager steps leaves the roles swapped, so up to two graphs are kept, one
+// per arrangement of the arrays.
 #include "ldg.h"
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <random>
 #include <string>
 #include <vector>
 
@@ -27,31 +36,75 @@ namespace {
 
 std::string g_create_error;
 
-// elements per workgroup for each order (LDS-limited, see kernel headers)
-template <int N> struct Tile;
-template <> struct Tile<2> { static constexpr int E = 16, EG = 16; };
-template <> struct Tile<3> { static constexpr int E = 8, EG = 8; };
-template <> struct Tile<4> { static constexpr int E = 4, EG = 4; };
-template <> struct Tile<5> { static constexpr int E = 2, EG = 2; };
-template <> struct Tile<6> { static constexpr int E = 1, EG = 1; };
+// Candidate elements per workgroup of each kernel, per order (the
+// autotuner picks among them; index 0 is the default).
+constexpr int kMaxCand = 2;
+template <int N> struct Cand;
+template <> struct Cand<2> { static constexpr int rhs[kMaxCand] = {16, 8}, grad[kMaxCand] = {16, 32}; };
+template <> struct Cand<3> { static constexpr int rhs[kMaxCand] = {8, 4}, grad[kMaxCand] = {8, 16}; };
+template <> struct Cand<4> { static constexpr int rhs[kMaxCand] = {4, 2}, grad[kMaxCand] = {4, 8}; };
+template <> struct Cand<5> { static constexpr int rhs[kMaxCand] = {2, 1}, grad[kMaxCand] = {2, 4}; };
+template <> struct Cand<6> { static constexpr int rhs[kMaxCand] = {1, 1}, grad[kMaxCand] = {1, 2}; };
+
+int cand_e(int n, bool rhs, int c) {
+  switch (n) {
+    case 2: return rhs ? Cand<2>::rhs[c] : Cand<2>::grad[c];
+    case 3: return rhs ? Cand<3>::rhs[c] : Cand<3>::grad[c];
+    case 4: return rhs ? Cand<4>::rhs[c] : Cand<4>::grad[c];
+    case 5: return rhs ? Cand<5>::rhs[c] : Cand<5>::grad[c];
+    case 6: return rhs ? Cand<6>::rhs[c] : Cand<6>::grad[c];
+  }
+  return 1;
+}
+
+struct Inst {
+  int p, nx, ny, nz;
+  double Re, M0, V0, phase[3];
+  double dx[3];
+  long K;
+  int group, slot;        // group and position inside it
+};
+
+// Per (group, candidate E): instance table and block -> instance map.
+struct Tiling {
+  int E = 1, nblocks = 0;
+  InstDesc *d_inst = nullptr;
+  int *d_blk = nullptr;   // nullptr when the group has one instance
+  std::vector<InstDesc> h_inst;
+};
+
+struct Group {
+  int n = 0;              // nodes per direction (p + 1)
+  std::vector<int> members;   // handle-level instance indices
+  long K = 0;
+  size_t nU = 0;
+  double *u = nullptr, *acc = nullptr, *ua = nullptr, *ub = nullptr, *k = nullptr;
+  void *q = nullptr;
+  double *ke_part = nullptr, *diag_part = nullptr, *d_scale = nullptr;
+  void *ops = nullptr;
+  Tiling rhs[kMaxCand], grad[kMaxCand];
+  int crhs = 0, cgrad = 0;   // chosen candidates
+  Basis1D basis;
+};
 
 }  // namespace
 
 struct ldg_handle {
   ldg_config cfg{};
-  int n = 0, dofs = 0;
-  long K = 0;
-  size_t nU = 0;
-  double dx[3]{};
-  Basis1D basis;
-  hipStream_t stream = nullptr;
-  double *u = nullptr, *acc = nullptr, *ua = nullptr, *ub = nullptr, *k = nullptr;
-  void *q = nullptr;     // gradient: double, or float when mixed
   bool mixed = false;
-  double *ke_part = nullptr, *diag_part = nullptr, *ke_hist = nullptr, *scal = nullptr;
-  void *ops = nullptr;   // Ops<N, double> in device memory
-  int nblocks_rhs = 0, nblocks_grad = 0;
+  std::vector<Inst> inst;
+  std::vector<Group> groups;
+  hipStream_t stream = nullptr;
+  double *ke_hist = nullptr, *scal = nullptr;
+  int *d_step = nullptr;
   size_t ke_hist_cap = 0;
+  std::vector<double> dt_cur;       // dt currently written into the instance tables
+  bool use_graph = false;
+  struct GraphSlot {
+    hipGraphExec_t exec = nullptr;
+    bool ke = false;
+    const double *u = nullptr;     // groups[0].u when captured
+  } graphs[2];
   std::string err;
 };
 
@@ -85,109 +138,23 @@ Ops<N, double> make_ops(const Basis1D &B) {
   return o;
 }
 
-MeshArgs mesh_args(const ldg_handle *h, int E) {
-  MeshArgs m;
-  m.nx = h->cfg.nx; m.ny = h->cfg.ny; m.nz = h->cfg.nz;
-  m.nelem = h->K;
-  m.nblocks = (int)((h->K + E - 1) / E);
-  m.per_xcd = (m.nblocks + 7) / 8;
-  for (int l = 0; l < 3; ++l) m.ialpha[l] = 2.0 / h->dx[l];
-  return m;
-}
-
-PhysArgs phys_args(const ldg_handle *h) {
+PhysArgs phys_args(const ldg_config &c, const Inst &in) {
   PhysArgs p;
-  p.gamma = h->cfg.gamma;
-  p.gm1 = h->cfg.gamma - 1.0;
-  p.mu = 1.0 / h->cfg.Re;
-  p.kappa = p.mu * h->cfg.gamma / h->cfg.Pr;
-  const double c0 = h->cfg.V0 / h->cfg.M0;
-  const double p0 = h->cfg.rho0 * c0 * c0 / h->cfg.gamma;   // solver.cpp:40-41
-  p.rho0 = h->cfg.rho0;
+  p.gamma = c.gamma;
+  p.gm1 = c.gamma - 1.0;
+  p.mu = 1.0 / in.Re;
+  p.kappa = p.mu * c.gamma / c.Pr;
+  const double c0 = in.V0 / in.M0;
+  const double p0 = c.rho0 * c0 * c0 / c.gamma;   // solver.cpp:40-41
+  p.rho0 = c.rho0;
   p.E0 = p0 / p.gm1;
   p.H0 = p.E0 + p0;
   return p;
 }
 
-template <int N>
-int launch_grad_t(ldg_handle *h, const double *U) {
-  constexpr int E = Tile<N>::EG;
-  MeshArgs m = mesh_args(h, E);
-  dim3 grid(8 * m.per_xcd), block(NT);
-  const Ops<N, double> *op = (const Ops<N, double> *)h->ops;
-  if (h->mixed)
-    hipLaunchKernelGGL((ldg_grad_kernel<N, E, float>), grid, block, 0, h->stream, U, (float *)h->q, m, op);
-  else
-    hipLaunchKernelGGL((ldg_grad_kernel<N, E, double>), grid, block, 0, h->stream, U, (double *)h->q, m, op);
-  HIPCHECK(h, hipGetLastError());
-  return LDG_OK;
-}
-
-template <int N>
-int launch_rhs_t(ldg_handle *h, const double *U, const StageArgs &st) {
-  constexpr int E = Tile<N>::E;
-  MeshArgs m = mesh_args(h, E);
-  dim3 grid(8 * m.per_xcd), block(NT);
-  const Ops<N, double> *op = (const Ops<N, double> *)h->ops;
-  if (h->mixed)
-    hipLaunchKernelGGL((ldg_rhs_kernel<N, E, float>), grid, block, 0, h->stream, U, (const float *)h->q, m, op,
-                       phys_args(h), st);
-  else
-    hipLaunchKernelGGL((ldg_rhs_kernel<N, E, double>), grid, block, 0, h->stream, U, (const double *)h->q, m, op,
-                       phys_args(h), st);
-  HIPCHECK(h, hipGetLastError());
-  return LDG_OK;
-}
-
-int launch_grad(ldg_handle *h, const double *U) {
-  switch (h->n) {
-    case 2: return launch_grad_t<2>(h, U);
-    case 3: return launch_grad_t<3>(h, U);
-    case 4: return launch_grad_t<4>(h, U);
-    case 5: return launch_grad_t<5>(h, U);
-    case 6: return launch_grad_t<6>(h, U);
-  }
-  return fail(h, LDG_EINVAL, "unsupported order");
-}
-
-int launch_rhs(ldg_handle *h, const double *U, const StageArgs &st) {
-  switch (h->n) {
-    case 2: return launch_rhs_t<2>(h, U, st);
-    case 3: return launch_rhs_t<3>(h, U, st);
-    case 4: return launch_rhs_t<4>(h, U, st);
-    case 5: return launch_rhs_t<5>(h, U, st);
-    case 6: return launch_rhs_t<6>(h, U, st);
-  }
-  return fail(h, LDG_EINVAL, "unsupported order");
-}
-
-template <int N>
-int upload_ops_t(ldg_handle *h) {
-  const Ops<N, double> o = make_ops<N>(h->basis);
-  HIPCHECK(h, hipMalloc(&h->ops, sizeof(o)));
-  HIPCHECK(h, hipMemcpy(h->ops, &o, sizeof(o), hipMemcpyHostToDevice));
-  return LDG_OK;
-}
-
-int upload_ops(ldg_handle *h) {
-  switch (h->n) {
-    case 2: return upload_ops_t<2>(h);
-    case 3: return upload_ops_t<3>(h);
-    case 4: return upload_ops_t<4>(h);
-    case 5: return upload_ops_t<5>(h);
-    case 6: return upload_ops_t<6>(h);
-  }
-  return fail(h, LDG_EINVAL, "unsupported order");
-}
-
-int rhs_blocks(int n, long K) {
-  int E = n == 2 ? Tile<2>::E : n == 3 ? Tile<3>::E : n == 4 ? Tile<4>::E : n == 5 ? Tile<5>::E : Tile<6>::E;
-  return (int)((K + E - 1) / E);
-}
-
-double ke_scale(const ldg_handle *h) {
-  const double J = (h->dx[0] / 2) * (h->dx[1] / 2) * (h->dx[2] / 2);
-  return J / (h->cfg.rho0 * std::pow(2 * M_PI * h->cfg.L, 3.0));
+double ke_scale(const ldg_config &c, const Inst &in) {
+  const double J = (in.dx[0] / 2) * (in.dx[1] / 2) * (in.dx[2] / 2);
+  return J / (c.rho0 * std::pow(2 * M_PI * c.L, 3.0));
 }
 
 StageArgs empty_stage() {
@@ -196,33 +163,406 @@ StageArgs empty_stage() {
   return s;
 }
 
-// One RK4 step of the resident state; KE of u^{n+1} to ke_out (device) if set.
-int rk4_step(ldg_handle *h, double dt, double *ke_out) {
+// ---------------------------------------------------------------- launches
+
+template <int N, int E>
+void launch_grad_e(ldg_handle *h, Group &g, const Tiling &t, const double *U) {
+  LaunchArgs la{t.d_inst, t.d_blk, t.nblocks, (t.nblocks + 7) / 8};
+  const Ops<N, double> *op = (const Ops<N, double> *)g.ops;
+  if (h->mixed)
+    hipLaunchKernelGGL((ldg_grad_kernel<N, E, float>), dim3(8 * la.per_xcd), dim3(NT), 0, h->stream, U,
+                       (float *)g.q, la, op);
+  else
+    hipLaunchKernelGGL((ldg_grad_kernel<N, E, double>), dim3(8 * la.per_xcd), dim3(NT), 0, h->stream, U,
+                       (double *)g.q, la, op);
+}
+
+template <int N, int E>
+void launch_rhs_e(ldg_handle *h, Group &g, const Tiling &t, const double *U, const StageArgs &st) {
+  LaunchArgs la{t.d_inst, t.d_blk, t.nblocks, (t.nblocks + 7) / 8};
+  const Ops<N, double> *op = (const Ops<N, double> *)g.ops;
+  if (h->mixed)
+    hipLaunchKernelGGL((ldg_rhs_kernel<N, E, float>), dim3(8 * la.per_xcd), dim3(NT), 0, h->stream, U,
+                       (const float *)g.q, la, op, st);
+  else
+    hipLaunchKernelGGL((ldg_rhs_kernel<N, E, double>), dim3(8 * la.per_xcd), dim3(NT), 0, h->stream, U,
+                       (const double *)g.q, la, op, st);
+}
+
+template <int N>
+void launch_grad_n(ldg_handle *h, Group &g, int c, const double *U) {
+  if (c == 0) launch_grad_e<N, Cand<N>::grad[0]>(h, g, g.grad[0], U);
+  else launch_grad_e<N, Cand<N>::grad[1]>(h, g, g.grad[1], U);
+}
+
+template <int N>
+void launch_rhs_n(ldg_handle *h, Group &g, int c, const double *U, const StageArgs &st) {
+  if (c == 0) launch_rhs_e<N, Cand<N>::rhs[0]>(h, g, g.rhs[0], U, st);
+  else launch_rhs_e<N, Cand<N>::rhs[1]>(h, g, g.rhs[1], U, st);
+}
+
+int launch_grad(ldg_handle *h, Group &g, const double *U, int c = -1) {
+  if (c < 0) c = g.cgrad;
+  switch (g.n) {
+    case 2: launch_grad_n<2>(h, g, c, U); break;
+    case 3: launch_grad_n<3>(h, g, c, U); break;
+    case 4: launch_grad_n<4>(h, g, c, U); break;
+    case 5: launch_grad_n<5>(h, g, c, U); break;
+    case 6: launch_grad_n<6>(h, g, c, U); break;
+    default: return fail(h, LDG_EINVAL, "unsupported order");
+  }
+  HIPCHECK(h, hipGetLastError());
+  return LDG_OK;
+}
+
+int launch_rhs(ldg_handle *h, Group &g, const double *U, const StageArgs &st, int c = -1) {
+  if (c < 0) c = g.crhs;
+  switch (g.n) {
+    case 2: launch_rhs_n<2>(h, g, c, U, st); break;
+    case 3: launch_rhs_n<3>(h, g, c, U, st); break;
+    case 4: launch_rhs_n<4>(h, g, c, U, st); break;
+    case 5: launch_rhs_n<5>(h, g, c, U, st); break;
+    case 6: launch_rhs_n<6>(h, g, c, U, st); break;
+    default: return fail(h, LDG_EINVAL, "unsupported order");
+  }
+  HIPCHECK(h, hipGetLastError());
+  return LDG_OK;
+}
+
+// ----------------------------------------------------------------- set-up
+
+template <int N>
+int upload_ops_t(ldg_handle *h, Group &g) {
+  const Ops<N, double> o = make_ops<N>(g.basis);
+  HIPCHECK(h, hipMalloc(&g.ops, sizeof(o)));
+  HIPCHECK(h, hipMemcpy(g.ops, &o, sizeof(o), hipMemcpyHostToDevice));
+  return LDG_OK;
+}
+
+int upload_ops(ldg_handle *h, Group &g) {
+  switch (g.n) {
+    case 2: return upload_ops_t<2>(h, g);
+    case 3: return upload_ops_t<3>(h, g);
+    case 4: return upload_ops_t<4>(h, g);
+    case 5: return upload_ops_t<5>(h, g);
+    case 6: return upload_ops_t<6>(h, g);
+  }
+  return fail(h, LDG_EINVAL, "unsupported order");
+}
+
+int build_tiling(ldg_handle *h, Group &g, Tiling &t, int E) {
+  t.E = E;
+  t.h_inst.clear();
+  std::vector<int> blk;
+  long off = 0;
+  int boff = 0;
+  for (int idx : g.members) {
+    const Inst &in = h->inst[idx];
+    InstDesc d;
+    std::memset(&d, 0, sizeof(d));
+    d.nx = in.nx; d.ny = in.ny; d.nz = in.nz;
+    d.nelem = in.K;
+    d.elem_off = off;
+    d.nblocks = (int)((in.K + E - 1) / E);
+    d.block_off = boff;
+    for (int l = 0; l < 3; ++l) d.ialpha[l] = 2.0 / in.dx[l];
+    d.dt = 0.0;
+    d.ph = phys_args(h->cfg, in);
+    for (int b = 0; b < d.nblocks; ++b) blk.push_back(in.slot);
+    off += in.K;
+    boff += d.nblocks;
+    t.h_inst.push_back(d);
+  }
+  t.nblocks = boff;
+  HIPCHECK(h, hipMalloc((void **)&t.d_inst, sizeof(InstDesc) * t.h_inst.size()));
+  HIPCHECK(h, hipMemcpy(t.d_inst, t.h_inst.data(), sizeof(InstDesc) * t.h_inst.size(), hipMemcpyHostToDevice));
+  if (g.members.size() > 1) {
+    HIPCHECK(h, hipMalloc((void **)&t.d_blk, sizeof(int) * blk.size()));
+    HIPCHECK(h, hipMemcpy(t.d_blk, blk.data(), sizeof(int) * blk.size(), hipMemcpyHostToDevice));
+  }
+  return LDG_OK;
+}
+
+void free_tiling(Tiling &t) {
+  if (t.d_inst) (void)hipFree(t.d_inst);
+  if (t.d_blk) (void)hipFree(t.d_blk);
+  t.d_inst = nullptr;
+  t.d_blk = nullptr;
+}
+
+int validate_instance(const ldg_config &c, const ldg_instance &in) {
+  if (in.p < 1 || in.p > 5) return fail(nullptr, LDG_EINVAL, "order p must be in 1..5");
+  if (in.nx < 3 || in.ny < 3 || in.nz < 3)
+    return fail(nullptr, LDG_EINVAL, "need at least 3 elements per direction (mesh.cpp:102-107)");
+  if (!(in.Re > 0) || !(in.M0 > 0) || !(in.V0 > 0)) return fail(nullptr, LDG_EINVAL, "Re, M0 and V0 must be positive");
+  for (int l = 0; l < 3; ++l)
+    if (!(c.xmax[l] > c.xmin[l])) return fail(nullptr, LDG_EINVAL, "empty domain");
+  if (!(c.Pr > 0) || !(c.gamma > 1) || !(c.rho0 > 0) || !(c.L > 0) || !(c.R > 0))
+    return fail(nullptr, LDG_EINVAL, "physics parameters must be positive (gamma > 1)");
+  if (c.precision != LDG_FP64 && c.precision != LDG_MIXED) return fail(nullptr, LDG_EINVAL, "unknown precision mode");
+  return LDG_OK;
+}
+
+int create_impl(const ldg_config *cfg, const ldg_instance *insts, int n, ldg_handle **out) {
+  if (!cfg || !insts || !out || n < 1) return fail(nullptr, LDG_EINVAL, "null argument or empty ensemble");
+  *out = nullptr;
+  for (int i = 0; i < n; ++i) {
+    int rc = validate_instance(*cfg, insts[i]);
+    if (rc) return rc;
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(nullptr, LDG_EDEVICE, "no HIP device");
+
+  ldg_handle *h = new ldg_handle();
+  h->cfg = *cfg;
+  h->mixed = cfg->precision == LDG_MIXED;
+  auto bad = [&](int code) { g_create_error = h->err; ldg_destroy(h); return code; };
+  for (int i = 0; i < n; ++i) {
+    Inst in;
+    in.p = insts[i].p; in.nx = insts[i].nx; in.ny = insts[i].ny; in.nz = insts[i].nz;
+    in.Re = insts[i].Re; in.M0 = insts[i].M0; in.V0 = insts[i].V0;
+    const int ns[3] = {in.nx, in.ny, in.nz};
+    for (int l = 0; l < 3; ++l) {
+      in.phase[l] = insts[i].phase[l];
+      in.dx[l] = (cfg->xmax[l] - cfg->xmin[l]) / ns[l];
+    }
+    in.K = (long)in.nx * in.ny * in.nz;
+    in.group = -1;
+    for (size_t gi = 0; gi < h->groups.size(); ++gi)
+      if (h->groups[gi].n == in.p + 1) in.group = (int)gi;
+    if (in.group < 0) {
+      h->groups.emplace_back();
+      h->groups.back().n = in.p + 1;
+      in.group = (int)h->groups.size() - 1;
+    }
+    Group &g = h->groups[in.group];
+    in.slot = (int)g.members.size();
+    g.members.push_back(i);
+    g.K += in.K;
+    h->inst.push_back(in);
+  }
+  h->dt_cur.assign(n, 0.0);
+  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+    h->err = "hipStreamCreate failed";
+    return bad(LDG_EDEVICE);
+  }
+  for (Group &g : h->groups) {
+    g.nU = (size_t)g.n * g.n * g.n * NSTATE * g.K;
+    g.basis = make_basis(g.n - 1);
+    const size_t bytes = g.nU * sizeof(double);
+    double **arrs[5] = {&g.u, &g.acc, &g.ua, &g.ub, &g.k};
+    for (double **p : arrs)
+      if (hipMalloc((void **)p, bytes) != hipSuccess) { h->err = "device allocation failed (state)"; return bad(LDG_ENOMEM); }
+    if (hipMalloc(&g.q, g.nU * 3 * (h->mixed ? sizeof(float) : sizeof(double))) != hipSuccess) {
+      h->err = "device allocation failed (gradient)";
+      return bad(LDG_ENOMEM);
+    }
+    int maxblk = 0;
+    for (int c = 0; c < kMaxCand; ++c) {
+      if (build_tiling(h, g, g.rhs[c], cand_e(g.n, true, c)) || build_tiling(h, g, g.grad[c], cand_e(g.n, false, c)))
+        return bad(LDG_EDEVICE);
+      maxblk = std::max(maxblk, g.rhs[c].nblocks);
+    }
+    std::vector<double> sc;
+    for (int idx : g.members) sc.push_back(ke_scale(h->cfg, h->inst[idx]));
+    if (hipMalloc((void **)&g.ke_part, sizeof(double) * (maxblk + 1)) != hipSuccess ||
+        hipMalloc((void **)&g.diag_part, sizeof(double) * 2 * (maxblk + 1)) != hipSuccess ||
+        hipMalloc((void **)&g.d_scale, sizeof(double) * sc.size()) != hipSuccess) {
+      h->err = "device allocation failed (partials)";
+      return bad(LDG_ENOMEM);
+    }
+    if (hipMemcpy(g.d_scale, sc.data(), sizeof(double) * sc.size(), hipMemcpyHostToDevice) != hipSuccess ||
+        upload_ops(h, g) != LDG_OK)
+      return bad(LDG_EDEVICE);
+    if (hipMemsetAsync(g.u, 0, bytes, h->stream) != hipSuccess) { h->err = "device initialisation failed"; return bad(LDG_EDEVICE); }
+  }
+  if (hipMalloc((void **)&h->scal, sizeof(double) * 3 * n) != hipSuccess ||
+      hipMalloc((void **)&h->d_step, sizeof(int)) != hipSuccess) {
+    h->err = "device allocation failed (scalars)";
+    return bad(LDG_ENOMEM);
+  }
+  if (hipStreamSynchronize(h->stream) != hipSuccess) { h->err = "device initialisation failed"; return bad(LDG_EDEVICE); }
+  *out = h;
+  return LDG_OK;
+}
+
+// ------------------------------------------------------------- time steps
+
+// Write per-instance dt into every instance table (when it changed).
+int set_dt(ldg_handle *h, const double *dt) {
+  bool changed = false;
+  for (size_t i = 0; i < h->inst.size(); ++i) changed |= (h->dt_cur[i] != dt[i]);
+  if (!changed) return LDG_OK;
+  for (size_t i = 0; i < h->inst.size(); ++i) h->dt_cur[i] = dt[i];
+  HIPCHECK(h, hipStreamSynchronize(h->stream));
+  for (Group &g : h->groups) {
+    Tiling *ts[2 * kMaxCand] = {&g.rhs[0], &g.rhs[1], &g.grad[0], &g.grad[1]};
+    for (Tiling *t : ts) {
+      for (size_t s = 0; s < g.members.size(); ++s) t->h_inst[s].dt = dt[g.members[s]];
+      HIPCHECK(h, hipMemcpy(t->d_inst, t->h_inst.data(), sizeof(InstDesc) * t->h_inst.size(), hipMemcpyHostToDevice));
+    }
+  }
+  return LDG_OK;
+}
+
+// One RK4 step of every group (eager launches).  With ke, each group's
+// kinetic energies of u^{n+1} go to row *d_step of the KE history.
+int rk4_step(ldg_handle *h, bool ke) {
   static const double b[4] = {1 / 6.0, 1 / 3.0, 1 / 3.0, 1 / 6.0};
   static const double a[3] = {0.5, 0.5, 1.0};
-  const double *U = h->u;
-  double *bufs[2] = {h->ua, h->ub};
-  for (int s = 0; s < 4; ++s) {
-    int rc = launch_grad(h, U);
-    if (rc) return rc;
-    StageArgs st = empty_stage();
-    st.u0 = h->u;
-    st.acc = h->acc;
-    st.acc_init = (s == 0);
-    st.dtb = dt * b[s];
-    if (s < 3) { st.unext = bufs[s & 1]; st.dta = dt * a[s]; }
-    if (s == 3 && ke_out) st.ke_part = h->ke_part;
-    rc = launch_rhs(h, U, st);
-    if (rc) return rc;
-    U = bufs[s & 1];
+  int first = 0;
+  for (Group &g : h->groups) {
+    const double *U = g.u;
+    double *bufs[2] = {g.ua, g.ub};
+    for (int s = 0; s < 4; ++s) {
+      int rc = launch_grad(h, g, U);
+      if (rc) return rc;
+      StageArgs st = empty_stage();
+      st.u0 = g.u;
+      st.acc = g.acc;
+      st.acc_init = (s == 0);
+      st.cb = b[s];
+      if (s < 3) { st.unext = bufs[s & 1]; st.ca = a[s]; }
+      if (s == 3 && ke) st.ke_part = g.ke_part;
+      rc = launch_rhs(h, g, U, st);
+      if (rc) return rc;
+      U = bufs[s & 1];
+    }
+    if (ke) {
+      const Tiling &t = g.rhs[g.crhs];
+      hipLaunchKernelGGL(ldg_finalize_kernel, dim3(g.members.size()), dim3(NT), 0, h->stream,
+                         (const double *)g.ke_part, 1, (const InstDesc *)t.d_inst, (int)h->inst.size(),
+                         (const double *)g.d_scale, h->ke_hist + first, (const int *)h->d_step);
+      HIPCHECK(h, hipGetLastError());
+    }
+    std::swap(g.u, g.acc);
+    first += (int)g.members.size();
   }
-  if (ke_out) {
-    hipLaunchKernelGGL(ldg_finalize_kernel, dim3(1), dim3(NT), 0, h->stream, (const double *)h->ke_part,
-                       h->nblocks_rhs, 1, ke_scale(h), ke_out);
+  if (ke) {
+    hipLaunchKernelGGL(ldg_advance_kernel, dim3(1), dim3(1), 0, h->stream, h->d_step);
     HIPCHECK(h, hipGetLastError());
   }
-  std::swap(h->u, h->acc);
   return LDG_OK;
+}
+
+void drop_graph(ldg_handle *h) {
+  for (auto &g : h->graphs) {
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    g = ldg_handle::GraphSlot();
+  }
+}
+
+// The graph for the current arrangement of the arrays, built on first use:
+// two eager steps recorded on the stream (the u/acc roles are the same after
+// two steps, so the graph can be replayed any number of times).
+int get_graph(ldg_handle *h, bool ke, hipGraphExec_t *out) {
+  const double *u = h->groups[0].u;
+  int slot = -1;
+  for (int i = 0; i < 2; ++i)
+    if (h->graphs[i].exec && h->graphs[i].u == u && h->graphs[i].ke == ke) { *out = h->graphs[i].exec; return LDG_OK; }
+  for (int i = 0; i < 2 && slot < 0; ++i)
+    if (h->graphs[i].u == u || !h->graphs[i].exec) slot = i;
+  if (slot < 0) slot = 0;
+  auto &gs = h->graphs[slot];
+  if (gs.exec) (void)hipGraphExecDestroy(gs.exec);
+  gs = ldg_handle::GraphSlot();
+  hipGraph_t graph = nullptr;
+  HIPCHECK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+  int rc = rk4_step(h, ke);
+  if (!rc) rc = rk4_step(h, ke);
+  hipError_t e = hipStreamEndCapture(h->stream, &graph);
+  if (rc) { if (graph) (void)hipGraphDestroy(graph); return rc; }
+  if (e != hipSuccess) return fail(h, LDG_EDEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+  hipGraphExec_t exec = nullptr;
+  e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (e != hipSuccess) return fail(h, LDG_EDEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+  // capturing recorded the launches without running them; the u/acc roles
+  // were swapped twice, i.e. they are back where they started
+  gs.exec = exec;
+  gs.ke = ke;
+  gs.u = u;
+  *out = exec;
+  return LDG_OK;
+}
+
+// Instance index at each position of the group-by-group numbering used by
+// the device-side reductions.
+std::vector<int> group_order(const ldg_handle *h) {
+  std::vector<int> order;
+  for (const Group &g : h->groups) order.insert(order.end(), g.members.begin(), g.members.end());
+  return order;
+}
+
+int step_impl(ldg_handle *h, const double *dt, int nsteps, double *ke) {
+  if (nsteps < 0) return fail(h, LDG_EINVAL, "bad step arguments");
+  for (size_t i = 0; i < h->inst.size(); ++i)
+    if (!(dt[i] > 0)) return fail(h, LDG_EINVAL, "bad step arguments");
+  const size_t ni = h->inst.size();
+  if (ke && (size_t)nsteps * ni > h->ke_hist_cap) {
+    if (h->ke_hist) (void)hipFree(h->ke_hist);
+    h->ke_hist = nullptr;
+    h->ke_hist_cap = 0;
+    HIPCHECK(h, hipMalloc((void **)&h->ke_hist, sizeof(double) * nsteps * ni));
+    h->ke_hist_cap = (size_t)nsteps * ni;
+    drop_graph(h);            // the graph captured the old history pointer
+  }
+  int rc = set_dt(h, dt);
+  if (rc) return rc;
+  const int zero = 0;
+  HIPCHECK(h, hipMemcpyAsync(h->d_step, &zero, sizeof(int), hipMemcpyHostToDevice, h->stream));
+  int done = 0;
+  if (h->use_graph && nsteps >= 2) {
+    hipGraphExec_t exec = nullptr;
+    rc = get_graph(h, ke != nullptr, &exec);
+    if (rc) return rc;
+    for (; done + 2 <= nsteps; done += 2) HIPCHECK(h, hipGraphLaunch(exec, h->stream));
+  }
+  for (; done < nsteps; ++done) {
+    rc = rk4_step(h, ke != nullptr);
+    if (rc) return rc;
+  }
+  if (ke && nsteps > 0)
+    HIPCHECK(h, hipMemcpyAsync(ke, h->ke_hist, sizeof(double) * nsteps * ni, hipMemcpyDeviceToHost, h->stream));
+  HIPCHECK(h, hipStreamSynchronize(h->stream));
+  if (ke && ni > 1) {
+    // the device history is numbered group by group; reorder each row to
+    // instance order
+    const std::vector<int> order = group_order(h);
+    std::vector<double> row(ni);
+    for (int s = 0; s < nsteps; ++s) {
+      double *r = ke + (size_t)s * ni;
+      for (size_t pos = 0; pos < ni; ++pos) row[order[pos]] = r[pos];
+      std::copy(row.begin(), row.end(), r);
+    }
+  }
+  if (ke)
+    for (size_t i = 0; i < (size_t)nsteps * ni; ++i)
+      if (!std::isfinite(ke[i])) return fail(h, LDG_ENONFINITE, "kinetic energy became non-finite");
+  return LDG_OK;
+}
+
+// Position of instance i in its group's arrays.
+size_t inst_offset(const ldg_handle *h, int i) {
+  const Inst &in = h->inst[i];
+  const Group &g = h->groups[in.group];
+  size_t off = 0;
+  for (int s = 0; s < in.slot; ++s) off += (size_t)h->inst[g.members[s]].K;
+  return off * g.n * g.n * g.n * NSTATE;
+}
+
+size_t inst_values(const ldg_handle *h, int i) {
+  const Inst &in = h->inst[i];
+  const int n = in.p + 1;
+  return (size_t)in.K * n * n * n * NSTATE;
+}
+
+ldg_instance default_instance(const ldg_config &c) {
+  ldg_instance in;
+  std::memset(&in, 0, sizeof(in));
+  in.p = c.p; in.nx = c.nx; in.ny = c.ny; in.nz = c.nz;
+  in.Re = c.Re; in.M0 = c.M0; in.V0 = c.V0;
+  return in;
 }
 
 }  // namespace
@@ -245,224 +585,312 @@ const char *ldg_last_error(const ldg_handle *h) { return h ? h->err.c_str() : g_
 
 int ldg_create(const ldg_config *cfg, ldg_handle **out) {
   if (!cfg || !out) return fail(nullptr, LDG_EINVAL, "null argument");
-  *out = nullptr;
-  if (cfg->p < 1 || cfg->p > 5) return fail(nullptr, LDG_EINVAL, "order p must be in 1..5");
-  if (cfg->nx < 3 || cfg->ny < 3 || cfg->nz < 3)
-    return fail(nullptr, LDG_EINVAL, "need at least 3 elements per direction (mesh.cpp:102-107)");
-  for (int l = 0; l < 3; ++l)
-    if (!(cfg->xmax[l] > cfg->xmin[l])) return fail(nullptr, LDG_EINVAL, "empty domain");
-  if (!(cfg->Re > 0) || !(cfg->Pr > 0) || !(cfg->gamma > 1) || !(cfg->M0 > 0) || !(cfg->rho0 > 0) ||
-      !(cfg->V0 > 0) || !(cfg->L > 0) || !(cfg->R > 0))
-    return fail(nullptr, LDG_EINVAL, "physics parameters must be positive (gamma > 1)");
-  if (cfg->precision != LDG_FP64 && cfg->precision != LDG_MIXED) return fail(nullptr, LDG_EINVAL, "unknown precision mode");
-  int ndev = 0;
-  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(nullptr, LDG_EDEVICE, "no HIP device");
+  const ldg_instance in = default_instance(*cfg);
+  return create_impl(cfg, &in, 1, out);
+}
 
-  ldg_handle *h = new ldg_handle();
-  h->cfg = *cfg;
-  h->n = cfg->p + 1;
-  h->dofs = h->n * h->n * h->n;
-  h->K = (long)cfg->nx * cfg->ny * cfg->nz;
-  h->nU = (size_t)h->dofs * NSTATE * h->K;
-  const int ns[3] = {cfg->nx, cfg->ny, cfg->nz};
-  for (int l = 0; l < 3; ++l) h->dx[l] = (cfg->xmax[l] - cfg->xmin[l]) / ns[l];
-  h->basis = make_basis(cfg->p);
-  h->mixed = cfg->precision == LDG_MIXED;
-  h->nblocks_rhs = rhs_blocks(h->n, h->K);
-  auto bad = [&](int code) { int c = code; g_create_error = h->err; ldg_destroy(h); return c; };
-  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
-    h->err = "hipStreamCreate failed";
-    return bad(LDG_EDEVICE);
+int ldg_ensemble_create(const ldg_config *base, const ldg_instance *inst, int n, ldg_handle **out) {
+  return create_impl(base, inst, n, out);
+}
+
+int ldg_ensemble_sample(unsigned long long seed, int n, ldg_instance *out) {
+  if (!out || n < 0) return LDG_EINVAL;
+  std::mt19937_64 rng(seed);
+  auto uni = [&]() { return (rng() >> 11) * (1.0 / 9007199254740992.0); };   // [0, 1)
+  const int sizes[4] = {8, 12, 16, 24};
+  for (int i = 0; i < n; ++i) {
+    ldg_instance &in = out[i];
+    std::memset(&in, 0, sizeof(in));
+    const int ne = sizes[rng() % 4];
+    in.nx = in.ny = in.nz = ne;
+    in.p = 2 + (int)(rng() % 3);
+    in.Re = 400.0 * std::exp(uni() * std::log(3200.0 / 400.0));
+    in.M0 = 0.05 + 0.25 * uni();
+    const double u1 = 1.0 - uni(), u2 = uni();                  // Box-Muller, u1 in (0, 1]
+    const double gauss = std::sqrt(-2.0 * std::log(u1)) * std::cos(2.0 * M_PI * u2);
+    in.V0 = 1.0 + 0.05 * gauss;
+    for (int l = 0; l < 3; ++l) in.phase[l] = 2.0 * M_PI * uni();
   }
-  const size_t bytes = h->nU * sizeof(double);
-  double **arrs[5] = {&h->u, &h->acc, &h->ua, &h->ub, &h->k};
-  for (double **p : arrs)
-    if (hipMalloc((void **)p, bytes) != hipSuccess) { h->err = "device allocation failed (state)"; return bad(LDG_ENOMEM); }
-  const size_t qbytes = h->nU * 3 * (h->mixed ? sizeof(float) : sizeof(double));
-  if (hipMalloc(&h->q, qbytes) != hipSuccess) { h->err = "device allocation failed (gradient)"; return bad(LDG_ENOMEM); }
-  if (hipMalloc((void **)&h->ke_part, sizeof(double) * (h->nblocks_rhs + 1)) != hipSuccess ||
-      hipMalloc((void **)&h->diag_part, sizeof(double) * 2 * (h->nblocks_rhs + 1)) != hipSuccess ||
-      hipMalloc((void **)&h->scal, sizeof(double) * 8) != hipSuccess) {
-    h->err = "device allocation failed (partials)";
-    return bad(LDG_ENOMEM);
-  }
-  if (upload_ops(h) != LDG_OK) {
-    g_create_error = h->err;
-    ldg_destroy(h);
-    return LDG_EDEVICE;
-  }
-  if (hipMemsetAsync(h->u, 0, bytes, h->stream) != hipSuccess || hipStreamSynchronize(h->stream) != hipSuccess) {
-    h->err = "device initialisation failed";
-    return bad(LDG_EDEVICE);
-  }
-  *out = h;
   return LDG_OK;
 }
 
 void ldg_destroy(ldg_handle *h) {
   if (!h) return;
   if (h->stream) (void)hipStreamSynchronize(h->stream);
-  double *ptrs[] = {h->u, h->acc, h->ua, h->ub, h->k, h->ke_part, h->diag_part, h->ke_hist, h->scal};
-  for (double *p : ptrs) if (p) (void)hipFree(p);
-  if (h->q) (void)hipFree(h->q);
-  if (h->ops) (void)hipFree(h->ops);
+  drop_graph(h);
+  for (Group &g : h->groups) {
+    double *ptrs[] = {g.u, g.acc, g.ua, g.ub, g.k, g.ke_part, g.diag_part, g.d_scale};
+    for (double *p : ptrs) if (p) (void)hipFree(p);
+    if (g.q) (void)hipFree(g.q);
+    if (g.ops) (void)hipFree(g.ops);
+    for (int c = 0; c < kMaxCand; ++c) { free_tiling(g.rhs[c]); free_tiling(g.grad[c]); }
+  }
+  if (h->ke_hist) (void)hipFree(h->ke_hist);
+  if (h->scal) (void)hipFree(h->scal);
+  if (h->d_step) (void)hipFree(h->d_step);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
 }
 
 int ldg_sizes(const ldg_handle *h, int *dofs, int *nelem) {
   if (!h) return LDG_EINVAL;
-  if (dofs) *dofs = h->dofs;
-  if (nelem) *nelem = (int)h->K;
+  const int n = h->inst[0].p + 1;
+  if (dofs) *dofs = n * n * n;
+  if (nelem) *nelem = (int)h->inst[0].K;
+  return LDG_OK;
+}
+
+int ldg_instance_count(const ldg_handle *h, int *n) {
+  if (!h || !n) return LDG_EINVAL;
+  *n = (int)h->inst.size();
+  return LDG_OK;
+}
+
+int ldg_instance_sizes(const ldg_handle *h, int i, int *p, int *dofs, int *nelem) {
+  if (!h || i < 0 || i >= (int)h->inst.size()) return LDG_EINVAL;
+  const int n = h->inst[i].p + 1;
+  if (p) *p = h->inst[i].p;
+  if (dofs) *dofs = n * n * n;
+  if (nelem) *nelem = (int)h->inst[i].K;
   return LDG_OK;
 }
 
 int ldg_initial_condition(ldg_handle *h) {
   if (!h) return LDG_EINVAL;
-  TGVArgs t;
-  std::memset(&t, 0, sizeof(t));
-  t.n = h->n; t.nx = h->cfg.nx; t.ny = h->cfg.ny; t.nz = h->cfg.nz; t.nelem = h->K;
-  for (int l = 0; l < 3; ++l) { t.xmin[l] = h->cfg.xmin[l]; t.xmax[l] = h->cfg.xmax[l]; }
-  for (int i = 0; i < h->n; ++i) t.r[i] = h->basis.r[i];
-  const double c0 = h->cfg.V0 / h->cfg.M0;
-  const double p0 = h->cfg.rho0 * c0 * c0 / h->cfg.gamma;     // solver.cpp:40-43
-  const double T0 = p0 / (h->cfg.R * h->cfg.rho0);
-  t.L = h->cfg.L; t.p0 = p0; t.rho0 = h->cfg.rho0; t.V0 = h->cfg.V0; t.RT0 = h->cfg.R * T0; t.gamma = h->cfg.gamma;
-  hipLaunchKernelGGL(ldg_tgv_kernel, dim3(2048), dim3(256), 0, h->stream, h->u, t);
-  HIPCHECK(h, hipGetLastError());
+  for (size_t i = 0; i < h->inst.size(); ++i) {
+    const Inst &in = h->inst[i];
+    Group &g = h->groups[in.group];
+    TGVArgs t;
+    std::memset(&t, 0, sizeof(t));
+    t.n = g.n; t.nx = in.nx; t.ny = in.ny; t.nz = in.nz; t.nelem = in.K;
+    for (int l = 0; l < 3; ++l) { t.xmin[l] = h->cfg.xmin[l]; t.xmax[l] = h->cfg.xmax[l]; t.phase[l] = in.phase[l]; }
+    for (int k = 0; k < g.n; ++k) t.r[k] = g.basis.r[k];
+    const double c0 = in.V0 / in.M0;
+    const double p0 = h->cfg.rho0 * c0 * c0 / h->cfg.gamma;     // solver.cpp:40-43
+    const double T0 = p0 / (h->cfg.R * h->cfg.rho0);
+    t.L = h->cfg.L; t.p0 = p0; t.rho0 = h->cfg.rho0; t.V0 = in.V0; t.RT0 = h->cfg.R * T0; t.gamma = h->cfg.gamma;
+    hipLaunchKernelGGL(ldg_tgv_kernel, dim3(2048), dim3(256), 0, h->stream, g.u + inst_offset(h, (int)i), t);
+    HIPCHECK(h, hipGetLastError());
+  }
   HIPCHECK(h, hipStreamSynchronize(h->stream));
   return LDG_OK;
 }
 
-int ldg_set_state(ldg_handle *h, const double *u) {
-  if (!h || !u) return LDG_EINVAL;
-  HIPCHECK(h, hipMemcpyAsync(h->u, u, h->nU * sizeof(double), hipMemcpyHostToDevice, h->stream));
+int ldg_instance_set_state(ldg_handle *h, int i, const double *u) {
+  if (!h || !u || i < 0 || i >= (int)h->inst.size()) return LDG_EINVAL;
+  Group &g = h->groups[h->inst[i].group];
+  HIPCHECK(h, hipMemcpyAsync(g.u + inst_offset(h, i), u, inst_values(h, i) * sizeof(double), hipMemcpyHostToDevice,
+                             h->stream));
   HIPCHECK(h, hipStreamSynchronize(h->stream));
   return LDG_OK;
 }
 
-int ldg_get_state(ldg_handle *h, double *u) {
-  if (!h || !u) return LDG_EINVAL;
-  HIPCHECK(h, hipMemcpyAsync(u, h->u, h->nU * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+int ldg_instance_get_state(ldg_handle *h, int i, double *u) {
+  if (!h || !u || i < 0 || i >= (int)h->inst.size()) return LDG_EINVAL;
+  Group &g = h->groups[h->inst[i].group];
+  HIPCHECK(h, hipMemcpyAsync(u, g.u + inst_offset(h, i), inst_values(h, i) * sizeof(double), hipMemcpyDeviceToHost,
+                             h->stream));
   HIPCHECK(h, hipStreamSynchronize(h->stream));
   return LDG_OK;
 }
+
+int ldg_set_state(ldg_handle *h, const double *u) { return ldg_instance_set_state(h, 0, u); }
+int ldg_get_state(ldg_handle *h, double *u) { return ldg_instance_get_state(h, 0, u); }
 
 int ldg_state_device_ptr(ldg_handle *h, double **dptr) {
   if (!h || !dptr) return LDG_EINVAL;
-  *dptr = h->u;
+  *dptr = h->groups[h->inst[0].group].u + inst_offset(h, 0);
+  return LDG_OK;
+}
+
+int ldg_instance_time_step(const ldg_handle *h, int i, double cfl, double *dt) {
+  if (!h || !dt || i < 0 || i >= (int)h->inst.size()) return LDG_EINVAL;
+  const Inst &in = h->inst[i];
+  const double m = std::fmin(std::fmin(in.dx[0], in.dx[1]), in.dx[2]);
+  *dt = cfl * m / (in.V0 * (2 * in.p + 1));     // solver.cpp:47
   return LDG_OK;
 }
 
 double ldg_time_step(const ldg_handle *h, double cfl) {
-  if (!h) return -1.0;
-  const double m = std::fmin(std::fmin(h->dx[0], h->dx[1]), h->dx[2]);
-  return cfl * m / (h->cfg.V0 * (2 * h->cfg.p + 1));
+  double dt = -1.0;
+  if (!h || ldg_instance_time_step(h, 0, cfl, &dt) != LDG_OK) return -1.0;
+  return dt;
 }
 
 int ldg_rhs(ldg_handle *h, const double *u, double *k, double *Du) {
   if (!h || !u || !k) return LDG_EINVAL;
-  const size_t bytes = h->nU * sizeof(double);
-  HIPCHECK(h, hipMemcpyAsync(h->ua, u, bytes, hipMemcpyHostToDevice, h->stream));
-  int rc = launch_grad(h, h->ua);
+  if (h->inst.size() != 1) return fail(h, LDG_EINVAL, "ldg_rhs needs a single-instance handle");
+  Group &g = h->groups[0];
+  const size_t bytes = g.nU * sizeof(double);
+  HIPCHECK(h, hipMemcpyAsync(g.ua, u, bytes, hipMemcpyHostToDevice, h->stream));
+  int rc = launch_grad(h, g, g.ua);
   if (rc) return rc;
   StageArgs st = empty_stage();
-  st.kout = h->k;
-  rc = launch_rhs(h, h->ua, st);
+  st.kout = g.k;
+  rc = launch_rhs(h, g, g.ua, st);
   if (rc) return rc;
-  HIPCHECK(h, hipMemcpyAsync(k, h->k, bytes, hipMemcpyDeviceToHost, h->stream));
+  HIPCHECK(h, hipMemcpyAsync(k, g.k, bytes, hipMemcpyDeviceToHost, h->stream));
   if (Du) {
-    std::vector<double> qh(h->nU * 3);
+    std::vector<double> qh(g.nU * 3);
     if (h->mixed) {
-      std::vector<float> qf(h->nU * 3);
-      HIPCHECK(h, hipMemcpyAsync(qf.data(), h->q, h->nU * 3 * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+      std::vector<float> qf(g.nU * 3);
+      HIPCHECK(h, hipMemcpyAsync(qf.data(), g.q, g.nU * 3 * sizeof(float), hipMemcpyDeviceToHost, h->stream));
       HIPCHECK(h, hipStreamSynchronize(h->stream));
       for (size_t i = 0; i < qf.size(); ++i) qh[i] = qf[i];
     } else {
-      HIPCHECK(h, hipMemcpyAsync(qh.data(), h->q, bytes * 3, hipMemcpyDeviceToHost, h->stream));
+      HIPCHECK(h, hipMemcpyAsync(qh.data(), g.q, bytes * 3, hipMemcpyDeviceToHost, h->stream));
       HIPCHECK(h, hipStreamSynchronize(h->stream));
     }
     // (node, s + 5 l, element) -> Dus(node, s, element, l)
-    const size_t dofs = h->dofs;
-    for (long e = 0; e < h->K; ++e)
+    const size_t dofs = (size_t)g.n * g.n * g.n;
+    const long K = g.K;
+    for (long e = 0; e < K; ++e)
       for (int l = 0; l < 3; ++l)
         for (int s = 0; s < NSTATE; ++s)
-          std::memcpy(Du + dofs * (s + NSTATE * (e + (size_t)h->K * l)),
-                      qh.data() + dofs * ((size_t)e * NGRAD + 5 * l + s), dofs * sizeof(double));
+          std::memcpy(Du + dofs * (s + NSTATE * (e + (size_t)K * l)), qh.data() + dofs * ((size_t)e * NGRAD + 5 * l + s),
+                      dofs * sizeof(double));
   }
   HIPCHECK(h, hipStreamSynchronize(h->stream));
   return LDG_OK;
 }
 
 int ldg_step(ldg_handle *h, double dt, int nsteps, double *ke) {
-  if (!h || nsteps < 0 || !(dt > 0)) return fail(h, LDG_EINVAL, "bad step arguments");
-  if (ke && (size_t)nsteps > h->ke_hist_cap) {
-    if (h->ke_hist) (void)hipFree(h->ke_hist);
-    h->ke_hist = nullptr;
-    HIPCHECK(h, hipMalloc((void **)&h->ke_hist, sizeof(double) * nsteps));
-    h->ke_hist_cap = nsteps;
-  }
-  for (int it = 0; it < nsteps; ++it) {
-    int rc = rk4_step(h, dt, ke ? h->ke_hist + it : nullptr);
+  if (!h) return fail(h, LDG_EINVAL, "bad step arguments");
+  if (h->inst.size() != 1) return fail(h, LDG_EINVAL, "ldg_step needs a single-instance handle (see ldg_ensemble_step)");
+  return step_impl(h, &dt, nsteps, ke);
+}
+
+int ldg_ensemble_step(ldg_handle *h, const double *dt, int nsteps, double *ke) {
+  if (!h || !dt) return fail(h, LDG_EINVAL, "bad step arguments");
+  return step_impl(h, dt, nsteps, ke);
+}
+
+int ldg_set_graph(ldg_handle *h, int enable) {
+  if (!h) return LDG_EINVAL;
+  h->use_graph = enable != 0;
+  if (!h->use_graph) drop_graph(h);
+  return LDG_OK;
+}
+
+int ldg_instance_diagnostics(ldg_handle *h, int i, double out[3]) {
+  if (!h || !out || i < 0 || i >= (int)h->inst.size()) return LDG_EINVAL;
+  const int ni = (int)h->inst.size();
+  // one zero-length RK epilogue per group (acc = u + 0 k) gives the nodal KE
+  // of u; the Gauss-point sums come from the volume phase of the same launch
+  int first = 0;
+  for (Group &g : h->groups) {
+    int rc = launch_grad(h, g, g.u);
     if (rc) return rc;
+    StageArgs st = empty_stage();
+    st.u0 = g.u;
+    st.acc = g.ua;
+    st.acc_init = 1;
+    st.cb = 0.0;
+    st.ke_part = g.ke_part;
+    st.diag_part = g.diag_part;
+    rc = launch_rhs(h, g, g.u, st);
+    if (rc) return rc;
+    const Tiling &t = g.rhs[g.crhs];
+    hipLaunchKernelGGL(ldg_finalize_kernel, dim3(g.members.size()), dim3(NT), 0, h->stream, (const double *)g.ke_part,
+                       1, (const InstDesc *)t.d_inst, ni, (const double *)g.d_scale, h->scal + first,
+                       (const int *)nullptr);
+    hipLaunchKernelGGL(ldg_finalize_kernel, dim3(g.members.size()), dim3(NT), 0, h->stream,
+                       (const double *)g.diag_part, 2, (const InstDesc *)t.d_inst, ni, (const double *)g.d_scale,
+                       h->scal + ni + 2 * first, (const int *)nullptr);
+    HIPCHECK(h, hipGetLastError());
+    first += (int)g.members.size();
   }
-  if (ke && nsteps > 0) {
-    HIPCHECK(h, hipMemcpyAsync(ke, h->ke_hist, sizeof(double) * nsteps, hipMemcpyDeviceToHost, h->stream));
-  }
+  std::vector<double> all(3 * ni);
+  HIPCHECK(h, hipMemcpyAsync(all.data(), h->scal, 3 * ni * sizeof(double), hipMemcpyDeviceToHost, h->stream));
   HIPCHECK(h, hipStreamSynchronize(h->stream));
-  if (ke)
-    for (int i = 0; i < nsteps; ++i)
-      if (!std::isfinite(ke[i])) return fail(h, LDG_ENONFINITE, "kinetic energy became non-finite");
+  // instances are numbered group by group in the arrays above
+  const std::vector<int> order = group_order(h);
+  for (int pos = 0; pos < ni; ++pos) {
+    if (order[pos] == i) {
+      out[0] = all[pos];
+      out[1] = all[ni + 2 * pos];
+      out[2] = all[ni + 2 * pos + 1];
+    }
+  }
+  for (int k = 0; k < 3; ++k)
+    if (!std::isfinite(out[k])) return fail(h, LDG_ENONFINITE, "diagnostic is non-finite");
   return LDG_OK;
 }
 
 int ldg_diagnostics(ldg_handle *h, double out[3]) {
   if (!h || !out) return LDG_EINVAL;
-  // reference KE of u: a zero-length RK epilogue (acc = u + 0 k) gives the nodal sum
-  int rc = launch_grad(h, h->u);
-  if (rc) return rc;
-  StageArgs st = empty_stage();
-  st.u0 = h->u;
-  st.acc = h->ua;
-  st.acc_init = 1;
-  st.dtb = 0.0;
-  st.ke_part = h->ke_part;
-  st.diag_part = h->diag_part;
-  rc = launch_rhs(h, h->u, st);
-  if (rc) return rc;
-  const double sc = ke_scale(h);
-  hipLaunchKernelGGL(ldg_finalize_kernel, dim3(1), dim3(NT), 0, h->stream, (const double *)h->ke_part,
-                     h->nblocks_rhs, 1, sc, h->scal);
-  hipLaunchKernelGGL(ldg_finalize_kernel, dim3(1), dim3(NT), 0, h->stream, (const double *)h->diag_part,
-                     h->nblocks_rhs, 2, sc, h->scal + 1);
-  HIPCHECK(h, hipGetLastError());
-  HIPCHECK(h, hipMemcpyAsync(out, h->scal, 3 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
-  HIPCHECK(h, hipStreamSynchronize(h->stream));
-  for (int i = 0; i < 3; ++i)
-    if (!std::isfinite(out[i])) return fail(h, LDG_ENONFINITE, "diagnostic is non-finite");
+  return ldg_instance_diagnostics(h, 0, out);
+}
+
+int ldg_autotune(ldg_handle *h, int *chosen) {
+  if (!h) return LDG_EINVAL;
+  drop_graph(h);
+  hipEvent_t e0, e1;
+  HIPCHECK(h, hipEventCreate(&e0));
+  HIPCHECK(h, hipEventCreate(&e1));
+  int gi = 0;
+  for (Group &g : h->groups) {
+    for (int kind = 0; kind < 2; ++kind) {     // 0: gradient, 1: right-hand side
+      float best = 1e30f;
+      int bestc = 0;
+      for (int c = 0; c < kMaxCand; ++c) {
+        if (c > 0 && cand_e(g.n, kind == 1, c) == cand_e(g.n, kind == 1, 0)) continue;
+        float ms = 0.f;
+        for (int rep = 0; rep < 4; ++rep) {    // first launch warms up, not timed
+          if (rep == 1) HIPCHECK(h, hipEventRecord(e0, h->stream));
+          int rc;
+          if (kind == 0) {
+            rc = launch_grad(h, g, g.u, c);
+          } else {
+            StageArgs st = empty_stage();
+            st.kout = g.k;
+            rc = launch_rhs(h, g, g.u, st, c);
+          }
+          if (rc) return rc;
+        }
+        HIPCHECK(h, hipEventRecord(e1, h->stream));
+        HIPCHECK(h, hipEventSynchronize(e1));
+        HIPCHECK(h, hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best) { best = ms; bestc = c; }
+      }
+      if (kind == 0) g.cgrad = bestc; else g.crhs = bestc;
+    }
+    if (chosen) {
+      chosen[3 * gi] = g.n - 1;
+      chosen[3 * gi + 1] = cand_e(g.n, false, g.cgrad);
+      chosen[3 * gi + 2] = cand_e(g.n, true, g.crhs);
+    }
+    ++gi;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
   return LDG_OK;
 }
 
 int ldg_kernel_times(ldg_handle *h, double dt, int nsteps, double ms[2]) {
   if (!h || !ms || nsteps < 1) return LDG_EINVAL;
+  if (h->inst.size() != 1) return fail(h, LDG_EINVAL, "ldg_kernel_times needs a single-instance handle");
   static const double b[4] = {1 / 6.0, 1 / 3.0, 1 / 3.0, 1 / 6.0};
   static const double a[3] = {0.5, 0.5, 1.0};
+  int rc = set_dt(h, &dt);
+  if (rc) return rc;
+  Group &g = h->groups[0];
   hipEvent_t ev[3];
   for (auto &e : ev) HIPCHECK(h, hipEventCreate(&e));
   double tot[2] = {0, 0};
   int count = 0;
   for (int it = 0; it < nsteps; ++it) {
-    const double *U = h->u;
-    double *bufs[2] = {h->ua, h->ub};
+    const double *U = g.u;
+    double *bufs[2] = {g.ua, g.ub};
     for (int s = 0; s < 4; ++s) {
       HIPCHECK(h, hipEventRecord(ev[0], h->stream));
-      int rc = launch_grad(h, U);
+      rc = launch_grad(h, g, U);
       if (rc) return rc;
       HIPCHECK(h, hipEventRecord(ev[1], h->stream));
       StageArgs st = empty_stage();
-      st.u0 = h->u; st.acc = h->acc; st.acc_init = (s == 0); st.dtb = dt * b[s];
-      if (s < 3) { st.unext = bufs[s & 1]; st.dta = dt * a[s]; }
-      rc = launch_rhs(h, U, st);
+      st.u0 = g.u; st.acc = g.acc; st.acc_init = (s == 0); st.cb = b[s];
+      if (s < 3) { st.unext = bufs[s & 1]; st.ca = a[s]; }
+      rc = launch_rhs(h, g, U, st);
       if (rc) return rc;
       HIPCHECK(h, hipEventRecord(ev[2], h->stream));
       HIPCHECK(h, hipEventSynchronize(ev[2]));
@@ -472,7 +900,7 @@ int ldg_kernel_times(ldg_handle *h, double dt, int nsteps, double ms[2]) {
       tot[0] += t0; tot[1] += t1; ++count;
       U = bufs[s & 1];
     }
-    std::swap(h->u, h->acc);
+    std::swap(g.u, g.acc);
   }
   for (auto &e : ev) (void)hipEventDestroy(e);
   ms[0] = tot[0] / count;

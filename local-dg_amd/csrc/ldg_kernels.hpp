@@ -63,19 +63,54 @@ struct StageArgs {
   double *acc;        // running u^n + dt sum b_s k_s
   double *unext;      // next stage input u^n + dt a_s k_s
   double *kout;       // raw k (parity / single RHS)
-  double dtb, dta;
-  int acc_init;       // 1: acc = u0 + dtb k   0: acc += dtb k
+  double cb, ca;      // RK4 coefficients b_s, a_s (times each instance's dt)
+  int acc_init;       // 1: acc = u0 + dt b k   0: acc += dt b k
   double *ke_part;    // per-workgroup partial sums of the reference KE of acc
   double *diag_part;  // per-workgroup partial sums (Gauss KE, enstrophy)
+};
+
+// One solver instance inside a launch.  A launch covers the elements of all
+// instances of one order (an ensemble); a single solver is a launch of one
+// instance.  Instances are stored back to back in the launch's arrays.
+struct InstDesc {
+  int nx, ny, nz;
+  int nblocks;           // its logical workgroups
+  long nelem;            // elements of this instance
+  long elem_off;         // its first element in the launch's arrays
+  int block_off;         // its first logical workgroup
+  int pad;
+  double ialpha[3];      // 2 / dx_l
+  double dt;             // its time step
+  PhysArgs ph;
+};
+
+struct LaunchArgs {
+  const InstDesc *inst;
+  const int *blk_inst;   // logical workgroup -> instance (nullptr: one instance)
+  int nblocks;           // logical workgroups of the launch
+  int per_xcd;           // ceil(nblocks / 8)
 };
 
 // Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a
 // contiguous range of element blocks so that face neighbours in x and y
 // share that XCD's L2.  Returns -1 for padding workgroups.
-__device__ inline int logical_block(const MeshArgs &m) {
+__device__ inline int logical_block(const LaunchArgs &la) {
   const int b = blockIdx.x;
-  const int lb = (b & 7) * m.per_xcd + (b >> 3);
-  return lb < m.nblocks ? lb : -1;
+  const int lb = (b & 7) * la.per_xcd + (b >> 3);
+  return lb < la.nblocks ? lb : -1;
+}
+
+// The instance of logical workgroup lb, its mesh, and the block's first
+// element (local to the instance) and element count.
+template <int E>
+__device__ inline const InstDesc &block_instance(const LaunchArgs &la, int lb, MeshArgs &m, long &kl, int &ne) {
+  const InstDesc &d = la.inst[la.blk_inst ? la.blk_inst[lb] : 0];
+  m.nx = d.nx; m.ny = d.ny; m.nz = d.nz;
+  m.nelem = d.nelem;
+  m.ialpha[0] = d.ialpha[0]; m.ialpha[1] = d.ialpha[1]; m.ialpha[2] = d.ialpha[2];
+  kl = (long)(lb - d.block_off) * E;
+  ne = (int)((d.nelem - kl) < E ? (d.nelem - kl) : E);
+  return d;
 }
 
 __device__ inline long neighbor(const MeshArgs &m, long K, int d, int dir) {
@@ -90,10 +125,10 @@ __device__ inline long neighbor(const MeshArgs &m, long K, int d, int dir) {
 // (the index arithmetic involves 64-bit division): nb[e][f] for faces
 // f = -x,+x,-y,+y,-z,+z.
 template <int E>
-__device__ inline void fill_neighbors(const MeshArgs &m, long k0, int ne, long (*nb)[6], int tid) {
+__device__ inline void fill_neighbors(const MeshArgs &m, long kl, long off, int ne, long (*nb)[6], int tid) {
   if (tid < ne * 6) {
     const int e = tid / 6, f = tid % 6;
-    nb[e][f] = neighbor(m, k0 + e, f >> 1, (f & 1) ? 1 : -1);
+    nb[e][f] = off + neighbor(m, kl + e, f >> 1, (f & 1) ? 1 : -1);
   }
 }
 
@@ -239,18 +274,21 @@ __device__ inline double block_sum(double v, double *sred) {
 // ---------------------------------------------------------------------
 template <int N, int E, typename TQ>
 __global__ void __launch_bounds__(NT) ldg_grad_kernel(const double *__restrict__ U, TQ *__restrict__ Q,
-                                                        MeshArgs m, const Ops<N, double> *__restrict__ opp) {
+                                                        LaunchArgs la, const Ops<N, double> *__restrict__ opp) {
   constexpr int NN = N * N * N, NF = N * N;
   __shared__ double su[E][NSTATE][NN];
   __shared__ double sj[E][3][NSTATE][NF];
   __shared__ double sD[N][N + 1];
   __shared__ long snb[E][6];
-  const int lb = logical_block(m);
+  const int lb = logical_block(la);
   if (lb < 0) return;
-  const long k0 = (long)lb * E;
-  const int ne = (int)((m.nelem - k0) < E ? (m.nelem - k0) : E);
+  MeshArgs m;
+  long kl;
+  int ne;
+  const InstDesc &d = block_instance<E>(la, lb, m, kl, ne);
+  const long k0 = d.elem_off + kl;    // first element of the block in the arrays
   const int tid = threadIdx.x;
-  fill_neighbors<E>(m, k0, ne, snb, tid);
+  fill_neighbors<E>(m, kl, d.elem_off, ne, snb, tid);
 
   if (tid < N * (N + 1)) {
     const int i = tid / (N + 1), j = tid % (N + 1);
@@ -520,8 +558,8 @@ struct Fp64Buf<N, E, T, false> { double v[1][1]; };
 
 template <int N, int E, typename T>
 __global__ void __launch_bounds__(NT, (RhsLds<N, T>::WAVES))
-ldg_rhs_kernel(const double *__restrict__ U, const T *__restrict__ Q, MeshArgs m,
-               const Ops<N, double> *__restrict__ opp, PhysArgs ph, StageArgs st) {
+ldg_rhs_kernel(const double *__restrict__ U, const T *__restrict__ Q, LaunchArgs la,
+               const Ops<N, double> *__restrict__ opp, StageArgs st) {
   using L = RhsLds<N, T>;
   constexpr int NN = L::NN, NF = L::NF, NV = L::NV, BUF = L::BUF;
   __shared__ __align__(16) T sbuf[E][BUF];
@@ -530,10 +568,15 @@ ldg_rhs_kernel(const double *__restrict__ U, const T *__restrict__ Q, MeshArgs m
   __shared__ double sred[NT];
   __shared__ SOps<N, T> sops;
   __shared__ long snb[E][6];
-  const int lb = logical_block(m);
+  const int lb = logical_block(la);
   if (lb < 0) return;
-  const long k0 = (long)lb * E;
-  const int ne = (int)((m.nelem - k0) < E ? (m.nelem - k0) : E);
+  MeshArgs m;
+  long kl;
+  int ne;
+  const InstDesc &d = block_instance<E>(la, lb, m, kl, ne);
+  const long k0 = d.elem_off + kl;    // first element of the block in the arrays
+  const PhysArgs ph = d.ph;
+  const double dtb = d.dt * st.cb, dta = d.dt * st.ca;
   const int tid = threadIdx.x;
   const SOps<N, T> *so = &sops;
   const Bg<T> bg(ph);
@@ -543,7 +586,7 @@ ldg_rhs_kernel(const double *__restrict__ U, const T *__restrict__ Q, MeshArgs m
     else return reinterpret_cast<double *>(&sbuf[e][0]);
   };
 
-  fill_neighbors<E>(m, k0, ne, snb, tid);
+  fill_neighbors<E>(m, kl, d.elem_off, ne, snb, tid);
   load_sops<N, T>(&sops, opp, tid);
   for (int w = tid; w < ne * NSTATE * NN; w += NT) (&sr[0][0][0])[w] = T(0);
   __syncthreads();
@@ -835,11 +878,11 @@ ldg_rhs_kernel(const double *__restrict__ U, const T *__restrict__ Q, MeshArgs m
         const long gi = gofs + w;
         if (st.kout) st.kout[gi] = k;
         if (st.acc) {
-          const double a = (st.acc_init ? ru[it] : ra[it]) + st.dtb * k;
+          const double a = (st.acc_init ? ru[it] : ra[it]) + dtb * k;
           st.acc[gi] = a;
           if (st.ke_part) md(e)[r] = a;
         }
-        if (st.unext) st.unext[gi] = ru[it] + st.dta * k;
+        if (st.unext) st.unext[gi] = ru[it] + dta * k;
       }
     }
   }
@@ -858,17 +901,29 @@ ldg_rhs_kernel(const double *__restrict__ U, const T *__restrict__ Q, MeshArgs m
   }
 }
 
-// Ordered sum of per-workgroup partials: out[j] = scale * sum_b part[b*stride + j].
-__global__ void __launch_bounds__(NT) ldg_finalize_kernel(const double *__restrict__ part, int nparts, int stride,
-                                                            double scale, double *__restrict__ out) {
+// Ordered per-instance sums of per-workgroup partials, one workgroup per
+// instance: out[(row * row_stride + i) * stride + j] = scale_i * (sum over
+// the instance's blocks b of part[b * stride + j]), row = *step (0 if null).
+__global__ void __launch_bounds__(NT) ldg_finalize_kernel(const double *__restrict__ part, int stride,
+                                                            const InstDesc *__restrict__ inst, int row_stride,
+                                                            const double *__restrict__ scale, double *__restrict__ out,
+                                                            const int *__restrict__ step) {
   __shared__ double sred[NT];
+  const int i = blockIdx.x;
+  const int b0 = inst[i].block_off, b1 = b0 + inst[i].nblocks;
+  const long row = step ? *step : 0;
   for (int j = 0; j < stride; ++j) {
     double v = 0.0;
-    for (int b = threadIdx.x; b < nparts; b += NT) v += part[(long)b * stride + j];
+    for (int b = b0 + threadIdx.x; b < b1; b += NT) v += part[(long)b * stride + j];
     const double t = block_sum(v, sred);
-    if (threadIdx.x == 0) out[j] = t * scale;
+    if (threadIdx.x == 0) out[(row * row_stride + i) * stride + j] = t * scale[i];
     __syncthreads();
   }
+}
+
+// Advances the history row written by ldg_finalize_kernel (one thread).
+__global__ void ldg_advance_kernel(int *step) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *step += 1;
 }
 
 // Taylor-Green initial state at the CGL nodes (solver.cpp:194-236).
@@ -878,6 +933,7 @@ struct TGVArgs {
   double xmin[3], xmax[3];
   double r[8];
   double L, p0, rho0, V0, RT0, gamma;
+  double phase[3];       // the field is evaluated at x + phase (0 for the reference case)
 };
 
 __global__ void ldg_tgv_kernel(double *__restrict__ u, TGVArgs t) {
@@ -895,7 +951,7 @@ __global__ void ldg_tgv_kernel(double *__restrict__ u, TGVArgs t) {
       const double hi = (ie[l] + 1) * (t.xmax[l] - t.xmin[l]) / ns[l] + t.xmin[l];
       X[l] = lo + .5 * (t.r[in[l]] + 1.0) * (hi - lo);
     }
-    const double x = X[0], y = X[1], z = X[2], L = t.L;
+    const double x = X[0] + t.phase[0], y = X[1] + t.phase[1], z = X[2] + t.phase[2], L = t.L;
     const double pr = t.p0 + (t.rho0 * t.V0 * t.V0 / 16.0) * (cos(2 * x / L) + cos(2 * y / L)) * (cos(2 * z / L) + 2);
     const double rho = pr / t.RT0;
     const double m1 = t.rho0 * t.V0 * sin(x / L) * cos(y / L) * cos(z / L);

@@ -30,6 +30,9 @@ EXPORTED = [
     "ldg_config_default", "ldg_create", "ldg_destroy", "ldg_last_error", "ldg_sizes",
     "ldg_initial_condition", "ldg_set_state", "ldg_get_state", "ldg_state_device_ptr",
     "ldg_time_step", "ldg_rhs", "ldg_step", "ldg_diagnostics", "ldg_kernel_times",
+    "ldg_ensemble_sample", "ldg_ensemble_create", "ldg_instance_count", "ldg_instance_sizes",
+    "ldg_instance_time_step", "ldg_instance_set_state", "ldg_instance_get_state",
+    "ldg_instance_diagnostics", "ldg_ensemble_step", "ldg_set_graph", "ldg_autotune",
 ]
 
 
@@ -40,6 +43,14 @@ class ldg_config(ctypes.Structure):
         ("Re", ctypes.c_double), ("Pr", ctypes.c_double), ("gamma", ctypes.c_double),
         ("M0", ctypes.c_double), ("rho0", ctypes.c_double), ("V0", ctypes.c_double),
         ("R", ctypes.c_double), ("precision", ctypes.c_int),
+    ]
+
+
+class ldg_instance(ctypes.Structure):
+    _fields_ = [
+        ("p", ctypes.c_int), ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
+        ("Re", ctypes.c_double), ("M0", ctypes.c_double), ("V0", ctypes.c_double),
+        ("phase", ctypes.c_double * 3),
     ]
 
 
@@ -79,6 +90,19 @@ def load_library(path=LIB_PATH):
     L.ldg_step.argtypes = [h, ctypes.c_double, ctypes.c_int, _dp]
     L.ldg_diagnostics.argtypes = [h, _dp]
     L.ldg_kernel_times.argtypes = [h, ctypes.c_double, ctypes.c_int, _dp]
+    ip = ctypes.POINTER(ctypes.c_int)
+    L.ldg_ensemble_sample.argtypes = [ctypes.c_ulonglong, ctypes.c_int, ctypes.POINTER(ldg_instance)]
+    L.ldg_ensemble_create.argtypes = [ctypes.POINTER(ldg_config), ctypes.POINTER(ldg_instance), ctypes.c_int,
+                                      ctypes.POINTER(h)]
+    L.ldg_instance_count.argtypes = [h, ip]
+    L.ldg_instance_sizes.argtypes = [h, ctypes.c_int, ip, ip, ip]
+    L.ldg_instance_time_step.argtypes = [h, ctypes.c_int, ctypes.c_double, _dp]
+    L.ldg_instance_set_state.argtypes = [h, ctypes.c_int, _dp]
+    L.ldg_instance_get_state.argtypes = [h, ctypes.c_int, _dp]
+    L.ldg_instance_diagnostics.argtypes = [h, ctypes.c_int, _dp]
+    L.ldg_ensemble_step.argtypes = [h, _dp, ctypes.c_int, _dp]
+    L.ldg_set_graph.argtypes = [h, ctypes.c_int]
+    L.ldg_autotune.argtypes = [h, ip]
     for name in EXPORTED:
         fn = getattr(L, name)
         if name not in ("ldg_destroy", "ldg_last_error", "ldg_time_step"):
@@ -176,6 +200,111 @@ class LDG:
         out = np.empty(2)
         self._check(self._lib.ldg_kernel_times(self._h, dt, nsteps, _p(out)))
         return {"grad_ms": out[0], "rhs_ms": out[1]}
+
+    def set_graph(self, enable=True):
+        """Replay steps from a HIP graph of two RK4 steps (ldg_set_graph)."""
+        self._check(self._lib.ldg_set_graph(self._h, int(bool(enable))))
+
+    def autotune(self):
+        """Pick the fastest tiling per kernel (ldg_autotune); returns
+        [(p, elements per workgroup of the gradient, of the rhs), ...]."""
+        out = (ctypes.c_int * 15)()
+        self._check(self._lib.ldg_autotune(self._h, out))
+        ng = len({p for p in getattr(self, "orders", [self.p])})
+        return [tuple(out[3 * g:3 * g + 3]) for g in range(ng)]
+
+
+def sample_instances(seed, n):
+    """n instance descriptions drawn by ldg_ensemble_sample (a list of dicts)."""
+    lib = load_library()
+    arr = (ldg_instance * max(n, 1))()
+    rc = lib.ldg_ensemble_sample(seed, n, arr)
+    if rc != LDG_OK:
+        raise LDGError(rc, "ldg_ensemble_sample")
+    return [{"p": a.p, "nx": a.nx, "ny": a.ny, "nz": a.nz, "Re": a.Re, "M0": a.M0, "V0": a.V0,
+             "phase": tuple(a.phase)} for a in arr[:n]]
+
+
+class Ensemble(LDG):
+    """Many independent Taylor-Green solvers advanced by shared kernel
+    launches (ldg_ensemble_create).  `instances` is a list of dicts with
+    keys p, nx (ny, nz default to nx), Re, M0, V0 and phase; the domain,
+    remaining physics and precision are shared."""
+
+    def __init__(self, instances, xmin=None, xmax=None, L=1.0, Pr=0.71, gamma=1.4, rho0=1.0, R=8.314,
+                 precision=LDG_FP64):
+        lib = load_library()
+        cfg = ldg_config()
+        lib.ldg_config_default(ctypes.byref(cfg))
+        xmin = [-math.pi * L] * 3 if xmin is None else list(xmin)
+        xmax = [math.pi * L] * 3 if xmax is None else list(xmax)
+        for l in range(3):
+            cfg.xmin[l], cfg.xmax[l] = xmin[l], xmax[l]
+        cfg.L, cfg.Pr, cfg.gamma, cfg.rho0, cfg.R, cfg.precision = L, Pr, gamma, rho0, R, precision
+        n = len(instances)
+        arr = (ldg_instance * max(n, 1))()
+        for a, d in zip(arr, instances):
+            a.p, a.nx = d["p"], d["nx"]
+            a.ny, a.nz = d.get("ny", d["nx"]), d.get("nz", d["nx"])
+            a.Re, a.M0, a.V0 = d.get("Re", 1600.0), d.get("M0", 0.1), d.get("V0", 1.0)
+            for l, v in enumerate(d.get("phase", (0.0, 0.0, 0.0))):
+                a.phase[l] = v
+        self._lib = lib
+        self._h = ctypes.c_void_p()
+        rc = lib.ldg_ensemble_create(ctypes.byref(cfg), arr, n, ctypes.byref(self._h))
+        if rc != LDG_OK:
+            raise LDGError(rc, lib.ldg_last_error(None).decode())
+        self.cfg = cfg
+        self.count = n
+        self.orders, self.dofs_i, self.K_i = [], [], []
+        for i in range(n):
+            p, d, k = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+            self._check(lib.ldg_instance_sizes(self._h, i, ctypes.byref(p), ctypes.byref(d), ctypes.byref(k)))
+            self.orders.append(p.value)
+            self.dofs_i.append(d.value)
+            self.K_i.append(k.value)
+        self.p, self.n, self.dofs, self.K = self.orders[0], self.orders[0] + 1, self.dofs_i[0], self.K_i[0]
+
+    def autotune(self):
+        out = (ctypes.c_int * 15)()
+        self._check(self._lib.ldg_autotune(self._h, out))
+        return [tuple(out[3 * g:3 * g + 3]) for g in range(len(dict.fromkeys(self.orders)))]
+
+    def instance_shape(self, i):
+        return (self.K_i[i], 5, self.dofs_i[i])
+
+    def total_dof(self):
+        return sum(k * d * 5 for k, d in zip(self.K_i, self.dofs_i))
+
+    def time_steps(self, cfl=0.01):
+        out = np.empty(self.count)
+        for i in range(self.count):
+            v = ctypes.c_double()
+            self._check(self._lib.ldg_instance_time_step(self._h, i, cfl, ctypes.byref(v)))
+            out[i] = v.value
+        return out
+
+    def set_instance_state(self, i, u):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        assert u.shape == self.instance_shape(i), (u.shape, self.instance_shape(i))
+        self._check(self._lib.ldg_instance_set_state(self._h, i, _p(u)))
+
+    def get_instance_state(self, i):
+        u = np.empty(self.instance_shape(i))
+        self._check(self._lib.ldg_instance_get_state(self._h, i, _p(u)))
+        return u
+
+    def instance_diagnostics(self, i):
+        out = np.empty(3)
+        self._check(self._lib.ldg_instance_diagnostics(self._h, i, _p(out)))
+        return {"ke_ref": out[0], "ke": out[1], "enstrophy": out[2]}
+
+    def step(self, dt, nsteps, record_ke=True):
+        """nsteps steps, instance i with dt[i]; returns KE (nsteps, count)."""
+        dt = np.ascontiguousarray(np.broadcast_to(np.asarray(dt, dtype=np.float64), (self.count,)))
+        ke = np.empty((nsteps, self.count)) if record_ke else None
+        self._check(self._lib.ldg_ensemble_step(self._h, _p(dt), nsteps, _p(ke) if record_ke and nsteps else None))
+        return ke
 
 
 class Point:

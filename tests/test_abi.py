@@ -60,3 +60,27 @@ def test_null_arguments_are_rejected():
 def test_mesh_rejects_thin_mesh():
     with pytest.raises(ValueError):
         ldg_amd.Mesh(2, 8, 8)
+
+
+def test_ensemble_sampler_is_deterministic_and_in_range():
+    """ldg_ensemble_sample is host code: same seed, same instances; the
+    documented ranges hold; different seeds differ."""
+    a = ldg_amd.sample_instances(11, 200)
+    assert a == ldg_amd.sample_instances(11, 200)
+    assert a != ldg_amd.sample_instances(12, 200)
+    assert {d["p"] for d in a} == {2, 3, 4}
+    assert {d["nx"] for d in a} == {8, 12, 16, 24}
+    assert all(d["nx"] == d["ny"] == d["nz"] for d in a)
+    assert all(400 <= d["Re"] <= 3200 and 0.05 <= d["M0"] <= 0.3 for d in a)
+    assert all(0.7 < d["V0"] < 1.3 for d in a)
+    assert all(0 <= ph < 2 * 3.141592653589794 for d in a for ph in d["phase"])
+    v0 = [d["V0"] for d in a]
+    assert abs(sum(v0) / len(v0) - 1) < 0.02
+
+
+def test_ensemble_create_rejects_bad_instance():
+    bad = [{"p": 2, "nx": 4, "Re": 1600.0, "M0": 0.1, "V0": 1.0}, {"p": 7, "nx": 4}]
+    with pytest.raises(ldg_amd.LDGError):
+        ldg_amd.Ensemble(bad)
+    with pytest.raises(ldg_amd.LDGError):
+        ldg_amd.Ensemble([{"p": 2, "nx": 2}])
