@@ -164,6 +164,71 @@ int ldg_set_graph(ldg_handle *h, int enable);
    the right-hand-side kernel): 3 ints per group, at most 5 groups. */
 int ldg_autotune(ldg_handle *h, int *chosen);
 
+/* ------------------------------------------------------------------------
+ * Learned closure for under-resolved runs.  No reference counterpart: the
+ * reference has no closure.  A per-element MLP maps one element's state and
+ * LDG gradient (20 fields at the (p+1)^3 nodes, 540 values at p = 2) to a
+ * correction of its right-hand side (5 fields, 135 values):
+ * in -> hidden -> hidden -> out with SiLU, FP32 weights entering BF16
+ * matrix-core products with FP32 accumulation.  It is trained on
+ * high-order runs of this solver: target = P rhs_fine(u) - rhs_coarse(P u)
+ * with P the element-wise L2 projection to the coarse order.
+ * ------------------------------------------------------------------------ */
+typedef struct ldg_closure ldg_closure;
+
+/* Closure for order p with `hidden` units per hidden layer and room for
+   `capacity` training samples; weights drawn from seed. */
+int ldg_closure_create(int p, int hidden, long capacity, unsigned long long seed, ldg_closure **out);
+void ldg_closure_destroy(ldg_closure *c);
+const char *ldg_closure_last_error(const ldg_closure *c);
+int ldg_closure_sizes(const ldg_closure *c, int *nin, int *nout, int *hidden, long *nparam);
+
+/* Parameters (nparam floats: W1, b1, W2, b2, W3, b3, each W row-major
+   out x in) and normalisation (2 (nin + nout) floats: input mean, input
+   1/std, output mean, output std).  Either pointer may be NULL.  Setting
+   parameters resets the optimiser state. */
+int ldg_closure_get(ldg_closure *c, float *params, float *norm);
+int ldg_closure_set(ldg_closure *c, const float *params, const float *norm);
+
+/* Network on B normalised rows (host arrays): Y (B x nout) = net(X);
+   ldg_closure_gradient: the gradient of mean((net(X) - T)^2) w.r.t. the
+   parameters into grad (nparam) and the loss. */
+int ldg_closure_forward(ldg_closure *c, const float *X, int B, float *Y);
+int ldg_closure_gradient(ldg_closure *c, const float *X, const float *T, int B, float *grad, double *loss);
+
+/* L2 projection of instance i's state of `fine` onto instance j of
+   `coarse` (same mesh, coarse order <= fine order), on the device. */
+int ldg_project_state(ldg_handle *fine, int i, ldg_handle *coarse, int j);
+
+/* Training samples from the resident states of `fine`: instance i of
+   `coarse` (the closure's order, same mesh) receives the projected state;
+   a fraction `keep` of the elements (chosen from seed) is appended as
+   (features, target) pairs.  Overwrites coarse's state. */
+int ldg_closure_collect(ldg_closure *c, ldg_handle *fine, ldg_handle *coarse, double keep, unsigned long long seed);
+/* Append n samples given as host arrays (raw, not normalised). */
+int ldg_closure_add_samples(ldg_closure *c, const float *X, const float *T, long n);
+int ldg_closure_samples(const ldg_closure *c, long *n);
+int ldg_closure_get_samples(ldg_closure *c, long first, long n, float *X, float *T);
+
+/* Per-feature mean/std of the collected samples become the normalisation;
+   the dataset is normalised in place (once). */
+int ldg_closure_fit_normalization(ldg_closure *c);
+
+/* Adam on minibatches of the first (1 - val_fraction) of the samples;
+   losses[2 e] / losses[2 e + 1] = training / validation MSE of epoch e
+   (normalised units). */
+int ldg_closure_train(ldg_closure *c, int epochs, int batch, double lr, double val_fraction, unsigned long long seed,
+                      double *losses);
+
+/* Attach c (NULL detaches) to every instance of h with the closure's
+   order: each RK stage (and ldg_rhs) adds the correction element by
+   element, unless the RMS of the normalised correction exceeds `bound`
+   (that element keeps the plain right-hand side; bound 0 switches every
+   element off).  c must outlive the attachment.  ldg_closure_gated: elements
+   switched off in the last evaluated stage, of `total`. */
+int ldg_set_closure(ldg_handle *h, const ldg_closure *c, double bound);
+int ldg_closure_gated(ldg_handle *h, long *off, long *total);
+
 #ifdef __cplusplus
 }
 #endif

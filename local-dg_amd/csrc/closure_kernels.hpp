@@ -1,0 +1,285 @@
+// closure_kernels.hpp -- device code of the learned per-element closure:
+// L2 projection between orders, feature/target assembly, a BF16-input MFMA
+// GEMM with fused bias + SiLU, and the elementwise pieces of the MLP's
+// backward pass and Adam.
+//
+// The network maps one element's state and LDG gradient (5 + 15 fields at
+// the (p+1)^3 nodes; 540 values at p = 2) to a correction of its
+// right-hand side (5 fields at the nodes; 135 values):
+//   H1 = silu(X W1^T + b1), H2 = silu(H1 W2^T + b2), Y = H2 W3^T + b3.
+// Weights are kept in FP32 and enter the matrix cores rounded to BF16, with
+// FP32 accumulation.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace ldg_mlp {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+
+// FP32 -> BF16, round to nearest even (inputs here are finite).
+__device__ inline unsigned short to_bf16(float f) {
+  unsigned u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (unsigned short)(u >> 16);
+}
+
+__device__ inline float silu(float z) { return z / (1.0f + __expf(-z)); }
+
+constexpr int GT = 64;       // output tile (rows and columns) per workgroup
+constexpr int GK = 32;       // K chunk staged in LDS
+constexpr int GPAD = 8;      // LDS row padding (BF16 elements)
+enum { EPI_NONE = 0, EPI_BIAS = 1, EPI_BIAS_SILU = 2 };
+
+// C[m][n] = epi( sum_k A(m,k) B(k,n) ), FP32 in memory, BF16 x BF16 -> FP32
+// on v_mfma_f32_32x32x16_bf16.  A(m,k) = TA ? A[k*lda+m] : A[m*lda+k];
+// B(k,n) = TB ? B[n*ldb+k] : B[k*ldb+n].  Epilogue: + bias[n]; with SiLU the
+// pre-activation goes to Z (if not null) and silu() to C.  Grid:
+// (ceil(N/64), ceil(M/64)), 256 threads; four waves own 32x32 quadrants.
+// Every global access is guarded by the matrix bounds (out-of-range
+// operands are staged as zeros).
+template <bool TA, bool TB, int EPI>
+__global__ void __launch_bounds__(256) mlp_gemm(int M, int N, int K, const float *__restrict__ A, int lda,
+                                                const float *__restrict__ B, int ldb, const float *__restrict__ bias,
+                                                float *__restrict__ Z, float *__restrict__ C, int ldc) {
+  __shared__ __align__(16) unsigned short As[GT][GK + GPAD];   // [m][k]
+  __shared__ __align__(16) unsigned short Bs[GT][GK + GPAD];   // [n][k]
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int m0 = blockIdx.y * GT, n0 = blockIdx.x * GT;
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  floatx16 acc;
+#pragma unroll
+  for (int j = 0; j < 16; ++j) acc[j] = 0.0f;
+
+  for (int k0 = 0; k0 < K; k0 += GK) {
+    if (!TA) {
+      const int r = tid >> 2, c = (tid & 3) * 8, m = m0 + r;
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        const int k = k0 + c + i;
+        As[r][c + i] = to_bf16((m < M && k < K) ? A[(long)m * lda + k] : 0.0f);
+      }
+    } else {
+      const int kk = tid >> 3, c = (tid & 7) * 8, k = k0 + kk;
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        const int m = m0 + c + i;
+        As[c + i][kk] = to_bf16((m < M && k < K) ? A[(long)k * lda + m] : 0.0f);
+      }
+    }
+    if (TB) {
+      const int r = tid >> 2, c = (tid & 3) * 8, n = n0 + r;
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        const int k = k0 + c + i;
+        Bs[r][c + i] = to_bf16((n < N && k < K) ? B[(long)n * ldb + k] : 0.0f);
+      }
+    } else {
+      const int kk = tid >> 3, c = (tid & 7) * 8, k = k0 + kk;
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        const int n = n0 + c + i;
+        Bs[c + i][kk] = to_bf16((n < N && k < K) ? B[(long)k * ldb + n] : 0.0f);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < GK; kk += 16) {
+      // lane l supplies row (l mod 32) of A / column (l mod 32) of B and the
+      // 8 consecutive k of half (l / 32) of the 16-deep step
+      const int kb = kk + (lane >> 5) * 8;
+      const u16x8 a = *reinterpret_cast<const u16x8 *>(&As[wm + (lane & 31)][kb]);
+      const u16x8 b = *reinterpret_cast<const u16x8 *>(&Bs[wn + (lane & 31)][kb]);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b),
+                                                    acc, 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  // accumulator j of lane l holds row 8 (j/4) + 4 (l/32) + j%4, column l%32
+  const int n = n0 + wn + (lane & 31);
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    const int m = m0 + wm + 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
+    if (m < M && n < N) {
+      float v = acc[j];
+      if (EPI >= EPI_BIAS) v += bias[n];
+      if (EPI == EPI_BIAS_SILU) {
+        if (Z) Z[(long)m * ldc + n] = v;
+        v = silu(v);
+      }
+      C[(long)m * ldc + n] = v;
+    }
+  }
+}
+
+// Features of elements [0, K) of one instance: X[e][j] = (x_j - mean_j) *
+// istd_j with x = (state minus the background (rho0, 0, 0, 0, E0): nU
+// values, then the gradient: nQ values), both in their per-element layout.
+template <typename TQ>
+__global__ void mlp_features(const double *__restrict__ U, const TQ *__restrict__ Q, long K, int dofs,
+                             double rho0, double E0, const float *__restrict__ mean, const float *__restrict__ istd,
+                             float *__restrict__ X) {
+  const int nU = 5 * dofs, nin = 20 * dofs;
+  const long total = K * nin;
+  for (long w = blockIdx.x * (long)blockDim.x + threadIdx.x; w < total; w += (long)gridDim.x * blockDim.x) {
+    const long e = w / nin;
+    const int j = (int)(w - e * nin);
+    double x;
+    if (j < nU) {
+      const int s = j / dofs;
+      x = U[e * nU + j] - (s == 0 ? rho0 : s == 4 ? E0 : 0.0);
+    } else {
+      x = (double)Q[e * (long)(15 * dofs) + (j - nU)];
+    }
+    X[w] = (float)((x - (double)mean[j]) * (double)istd[j]);
+  }
+}
+
+// Training targets of elements [0, K): T = Pk - k (projected fine
+// right-hand side minus the coarse right-hand side of the projected state).
+__global__ void mlp_targets(const double *__restrict__ Pk, const double *__restrict__ k, long n,
+                            float *__restrict__ T) {
+  for (long w = blockIdx.x * (long)blockDim.x + threadIdx.x; w < n; w += (long)gridDim.x * blockDim.x)
+    T[w] = (float)(Pk[w] - k[w]);
+}
+
+// Closure correction of each element (one 64-thread block per element):
+// corr = Y * ostd + omean, applied only if the RMS of the normalised
+// output Y is at most `bound` (gate = 1), else gate = 0 (plain RHS).
+__global__ void __launch_bounds__(64) mlp_correction(const float *__restrict__ Y, long K, int nout,
+                                                     const float *__restrict__ omean, const float *__restrict__ ostd,
+                                                     double bound, double *__restrict__ corr,
+                                                     unsigned char *__restrict__ gate) {
+  for (long e = blockIdx.x; e < K; e += gridDim.x) {
+    const float *y = Y + e * nout;
+    float ss = 0.0f;
+    for (int j = threadIdx.x; j < nout; j += 64) ss += y[j] * y[j];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) ss += __shfl_xor(ss, o, 64);
+    const bool on = sqrt((double)ss / nout) <= bound;
+    for (int j = threadIdx.x; j < nout; j += 64)
+      corr[e * nout + j] = on ? (double)y[j] * (double)ostd[j] + (double)omean[j] : 0.0;
+    if (threadIdx.x == 0) gate[e] = on ? 1 : 0;
+  }
+}
+
+// dY = scale (Y - T); per-block partial sums of (Y - T)^2 (FP64).
+__global__ void __launch_bounds__(256) mlp_loss_grad(const float *__restrict__ Y, const float *__restrict__ T,
+                                                     long n, float scale, float *__restrict__ dY,
+                                                     double *__restrict__ part) {
+  __shared__ double red[256];
+  double s = 0.0;
+  for (long w = blockIdx.x * 256L + threadIdx.x; w < n; w += (long)gridDim.x * 256) {
+    const float d = Y[w] - T[w];
+    if (dY) dY[w] = scale * d;
+    s += (double)d * d;
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// dZ = dH * silu'(Z), silu'(z) = s (1 + z (1 - s)), s = sigmoid(z).
+__global__ void mlp_silu_back(const float *__restrict__ dH, const float *__restrict__ Z, long n,
+                              float *__restrict__ dZ) {
+  for (long w = blockIdx.x * (long)blockDim.x + threadIdx.x; w < n; w += (long)gridDim.x * blockDim.x) {
+    const float z = Z[w];
+    const float s = 1.0f / (1.0f + __expf(-z));
+    dZ[w] = dH[w] * s * (1.0f + z * (1.0f - s));
+  }
+}
+
+// Column sums of a rows x cols row-major matrix (one thread per column,
+// fixed order: deterministic).
+__global__ void mlp_colsum(const float *__restrict__ D, int rows, int cols, float *__restrict__ out) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= cols) return;
+  float s = 0.0f;
+  for (int r = 0; r < rows; ++r) s += D[(long)r * cols + c];
+  out[c] = s;
+}
+
+// Adam (bias-corrected), elementwise over all parameters.
+__global__ void mlp_adam(float *__restrict__ p, const float *__restrict__ g, float *__restrict__ m,
+                         float *__restrict__ v, long n, float lr, float b1, float b2, float eps, float bc1, float bc2) {
+  for (long w = blockIdx.x * (long)blockDim.x + threadIdx.x; w < n; w += (long)gridDim.x * blockDim.x) {
+    const float gi = g[w];
+    const float mi = b1 * m[w] + (1.0f - b1) * gi;
+    const float vi = b2 * v[w] + (1.0f - b2) * gi * gi;
+    m[w] = mi;
+    v[w] = vi;
+    p[w] -= lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
+  }
+}
+
+// dst[b][:] = src[idx[b]][:] (rows of `width` floats).
+__global__ void mlp_gather(const float *__restrict__ src, int width, const long *__restrict__ idx, int rows,
+                           float *__restrict__ dst) {
+  const long total = (long)rows * width;
+  for (long w = blockIdx.x * (long)blockDim.x + threadIdx.x; w < total; w += (long)gridDim.x * blockDim.x) {
+    const long b = w / width;
+    const int j = (int)(w - b * width);
+    dst[w] = src[idx[b] * width + j];
+  }
+}
+
+// Per-feature sums over a chunk of rows (FP64), for mean and spread:
+// part[(chunk * 2 + 0) * width + j] = sum x, [... + 1] = sum x^2.
+__global__ void mlp_moments(const float *__restrict__ D, long rows, int width, long chunk, double *__restrict__ part) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= width) return;
+  const long r0 = blockIdx.y * chunk;
+  const long r1 = r0 + chunk < rows ? r0 + chunk : rows;
+  double s = 0.0, s2 = 0.0;
+  for (long r = r0; r < r1; ++r) {
+    const double x = D[r * width + j];
+    s += x;
+    s2 += x * x;
+  }
+  part[(blockIdx.y * 2L + 0) * width + j] = s;
+  part[(blockIdx.y * 2L + 1) * width + j] = s2;
+}
+
+// D[r][j] = (D[r][j] - mean[j]) * scale[j], in place.
+__global__ void mlp_normalize(float *__restrict__ D, long rows, int width, const float *__restrict__ mean,
+                              const float *__restrict__ scale) {
+  const long total = rows * width;
+  for (long w = blockIdx.x * (long)blockDim.x + threadIdx.x; w < total; w += (long)gridDim.x * blockDim.x) {
+    const int j = (int)(w % width);
+    D[w] = (D[w] - mean[j]) * scale[j];
+  }
+}
+
+// L2 projection between orders, element by element: dst (nc^3 nodes per
+// field) = (P (x) P (x) P) src (nf^3 nodes), P = nc x nf row-major, for 5
+// fields per element.  One thread per (element, field, coarse node).
+__global__ void mlp_project(int nf, int nc, long K, const double *__restrict__ P, const double *__restrict__ src,
+                            double *__restrict__ dst) {
+  const int df = nf * nf * nf, dc = nc * nc * nc;
+  const long total = K * 5 * dc;
+  for (long w = blockIdx.x * (long)blockDim.x + threadIdx.x; w < total; w += (long)gridDim.x * blockDim.x) {
+    const long es = w / dc;                 // element * 5 + field
+    const int node = (int)(w - es * dc);
+    const int i = node % nc, j = (node / nc) % nc, k = node / (nc * nc);
+    const double *s = src + es * df;
+    double acc = 0.0;
+    for (int c = 0; c < nf; ++c) {
+      double accb = 0.0;
+      for (int b = 0; b < nf; ++b) {
+        double acca = 0.0;
+        for (int a = 0; a < nf; ++a) acca += P[i * nf + a] * s[a + nf * (b + nf * c)];
+        accb += P[j * nf + b] * acca;
+      }
+      acc += P[k * nf + c] * accb;
+    }
+    dst[w] = acc;
+  }
+}
+
+}  // namespace ldg_mlp

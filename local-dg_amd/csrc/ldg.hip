@@ -27,18 +27,21 @@
 #include <string>
 #include <vector>
 
-#include "basis.hpp"
-#include "ldg_kernels.hpp"
+#include "handle.hpp"
 
 using namespace ldg;
 
-namespace {
+namespace ldg_host {
 
 std::string g_create_error;
 
+int fail(ldg_handle *h, int code, const std::string &msg) {
+  if (h) h->err = msg; else g_create_error = msg;
+  return code;
+}
+
 // Candidate elements per workgroup of each kernel, per order (the
 // autotuner picks among them; index 0 is the default).
-constexpr int kMaxCand = 2;
 template <int N> struct Cand;
 template <> struct Cand<2> { static constexpr int rhs[kMaxCand] = {16, 8}, grad[kMaxCand] = {16, 32}; };
 template <> struct Cand<3> { static constexpr int rhs[kMaxCand] = {8, 4}, grad[kMaxCand] = {8, 16}; };
@@ -56,71 +59,6 @@ int cand_e(int n, bool rhs, int c) {
   }
   return 1;
 }
-
-struct Inst {
-  int p, nx, ny, nz;
-  double Re, M0, V0, phase[3];
-  double dx[3];
-  long K;
-  int group, slot;        // group and position inside it
-};
-
-// Per (group, candidate E): instance table and block -> instance map.
-struct Tiling {
-  int E = 1, nblocks = 0;
-  InstDesc *d_inst = nullptr;
-  int *d_blk = nullptr;   // nullptr when the group has one instance
-  std::vector<InstDesc> h_inst;
-};
-
-struct Group {
-  int n = 0;              // nodes per direction (p + 1)
-  std::vector<int> members;   // handle-level instance indices
-  long K = 0;
-  size_t nU = 0;
-  double *u = nullptr, *acc = nullptr, *ua = nullptr, *ub = nullptr, *k = nullptr;
-  void *q = nullptr;
-  double *ke_part = nullptr, *diag_part = nullptr, *d_scale = nullptr;
-  void *ops = nullptr;
-  Tiling rhs[kMaxCand], grad[kMaxCand];
-  int crhs = 0, cgrad = 0;   // chosen candidates
-  Basis1D basis;
-};
-
-}  // namespace
-
-struct ldg_handle {
-  ldg_config cfg{};
-  bool mixed = false;
-  std::vector<Inst> inst;
-  std::vector<Group> groups;
-  hipStream_t stream = nullptr;
-  double *ke_hist = nullptr, *scal = nullptr;
-  int *d_step = nullptr;
-  size_t ke_hist_cap = 0;
-  std::vector<double> dt_cur;       // dt currently written into the instance tables
-  bool use_graph = false;
-  struct GraphSlot {
-    hipGraphExec_t exec = nullptr;
-    bool ke = false;
-    const double *u = nullptr;     // groups[0].u when captured
-  } graphs[2];
-  std::string err;
-};
-
-namespace {
-
-int fail(ldg_handle *h, int code, const std::string &msg) {
-  if (h) h->err = msg; else g_create_error = msg;
-  return code;
-}
-
-#define HIPCHECK(h, call)                                                               \
-  do {                                                                                  \
-    hipError_t e_ = (call);                                                             \
-    if (e_ != hipSuccess)                                                               \
-      return fail((h), LDG_EDEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
-  } while (0)
 
 template <int N>
 Ops<N, double> make_ops(const Basis1D &B) {
@@ -201,7 +139,7 @@ void launch_rhs_n(ldg_handle *h, Group &g, int c, const double *U, const StageAr
   else launch_rhs_e<N, Cand<N>::rhs[1]>(h, g, g.rhs[1], U, st);
 }
 
-int launch_grad(ldg_handle *h, Group &g, const double *U, int c = -1) {
+int launch_grad(ldg_handle *h, Group &g, const double *U, int c) {
   if (c < 0) c = g.cgrad;
   switch (g.n) {
     case 2: launch_grad_n<2>(h, g, c, U); break;
@@ -215,7 +153,7 @@ int launch_grad(ldg_handle *h, Group &g, const double *U, int c = -1) {
   return LDG_OK;
 }
 
-int launch_rhs(ldg_handle *h, Group &g, const double *U, const StageArgs &st, int c = -1) {
+int launch_rhs(ldg_handle *h, Group &g, const double *U, const StageArgs &st, int c) {
   if (c < 0) c = g.crhs;
   switch (g.n) {
     case 2: launch_rhs_n<2>(h, g, c, U, st); break;
@@ -424,6 +362,8 @@ int rk4_step(ldg_handle *h, bool ke) {
       st.cb = b[s];
       if (s < 3) { st.unext = bufs[s & 1]; st.ca = a[s]; }
       if (s == 3 && ke) st.ke_part = g.ke_part;
+      rc = closure_stage(h, g, U, st);
+      if (rc) return rc;
       rc = launch_rhs(h, g, U, st);
       if (rc) return rc;
       U = bufs[s & 1];
@@ -565,7 +505,9 @@ ldg_instance default_instance(const ldg_config &c) {
   return in;
 }
 
-}  // namespace
+}  // namespace ldg_host
+
+using namespace ldg_host;
 
 extern "C" {
 
@@ -624,6 +566,7 @@ void ldg_destroy(ldg_handle *h) {
     if (g.q) (void)hipFree(g.q);
     if (g.ops) (void)hipFree(g.ops);
     for (int c = 0; c < kMaxCand; ++c) { free_tiling(g.rhs[c]); free_tiling(g.grad[c]); }
+    closure_release(g);
   }
   if (h->ke_hist) (void)hipFree(h->ke_hist);
   if (h->scal) (void)hipFree(h->scal);
@@ -727,6 +670,8 @@ int ldg_rhs(ldg_handle *h, const double *u, double *k, double *Du) {
   if (rc) return rc;
   StageArgs st = empty_stage();
   st.kout = g.k;
+  rc = closure_stage(h, g, g.ua, st);
+  if (rc) return rc;
   rc = launch_rhs(h, g, g.ua, st);
   if (rc) return rc;
   HIPCHECK(h, hipMemcpyAsync(k, g.k, bytes, hipMemcpyDeviceToHost, h->stream));

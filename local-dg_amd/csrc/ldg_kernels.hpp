@@ -67,6 +67,8 @@ struct StageArgs {
   int acc_init;       // 1: acc = u0 + dt b k   0: acc += dt b k
   double *ke_part;    // per-workgroup partial sums of the reference KE of acc
   double *diag_part;  // per-workgroup partial sums (Gauss KE, enstrophy)
+  const double *corr;          // learned closure: k += corr where gate[element] != 0
+  const unsigned char *gate;
 };
 
 // One solver instance inside a launch.  A launch covers the elements of all
@@ -866,6 +868,7 @@ ldg_rhs_kernel(const double *__restrict__ U, const T *__restrict__ Q, LaunchArgs
         if (need_acc) ra[it] = st.acc[gofs + w];
         const int e = w / (NSTATE * NN), r = w - e * (NSTATE * NN);
         kv[it] = md(e)[r];
+        if (st.corr && st.gate[k0 + e]) kv[it] += st.corr[gofs + w];
       }
     }
     if (st.ke_part) __syncthreads();   // every k read before the new state reuses the buffer
@@ -904,7 +907,7 @@ ldg_rhs_kernel(const double *__restrict__ U, const T *__restrict__ Q, LaunchArgs
 // Ordered per-instance sums of per-workgroup partials, one workgroup per
 // instance: out[(row * row_stride + i) * stride + j] = scale_i * (sum over
 // the instance's blocks b of part[b * stride + j]), row = *step (0 if null).
-__global__ void __launch_bounds__(NT) ldg_finalize_kernel(const double *__restrict__ part, int stride,
+static __global__ void __launch_bounds__(NT) ldg_finalize_kernel(const double *__restrict__ part, int stride,
                                                             const InstDesc *__restrict__ inst, int row_stride,
                                                             const double *__restrict__ scale, double *__restrict__ out,
                                                             const int *__restrict__ step) {
@@ -922,7 +925,7 @@ __global__ void __launch_bounds__(NT) ldg_finalize_kernel(const double *__restri
 }
 
 // Advances the history row written by ldg_finalize_kernel (one thread).
-__global__ void ldg_advance_kernel(int *step) {
+static __global__ void ldg_advance_kernel(int *step) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *step += 1;
 }
 
@@ -936,7 +939,7 @@ struct TGVArgs {
   double phase[3];       // the field is evaluated at x + phase (0 for the reference case)
 };
 
-__global__ void ldg_tgv_kernel(double *__restrict__ u, TGVArgs t) {
+static __global__ void ldg_tgv_kernel(double *__restrict__ u, TGVArgs t) {
   const int dofs = t.n * t.n * t.n;
   const long total = t.nelem * dofs;
   for (long w = blockIdx.x * (long)blockDim.x + threadIdx.x; w < total; w += (long)gridDim.x * blockDim.x) {

@@ -33,6 +33,11 @@ EXPORTED = [
     "ldg_ensemble_sample", "ldg_ensemble_create", "ldg_instance_count", "ldg_instance_sizes",
     "ldg_instance_time_step", "ldg_instance_set_state", "ldg_instance_get_state",
     "ldg_instance_diagnostics", "ldg_ensemble_step", "ldg_set_graph", "ldg_autotune",
+    "ldg_closure_create", "ldg_closure_destroy", "ldg_closure_last_error", "ldg_closure_sizes",
+    "ldg_closure_get", "ldg_closure_set", "ldg_closure_forward", "ldg_closure_gradient",
+    "ldg_project_state", "ldg_closure_collect", "ldg_closure_add_samples", "ldg_closure_samples",
+    "ldg_closure_get_samples", "ldg_closure_fit_normalization", "ldg_closure_train", "ldg_set_closure",
+    "ldg_closure_gated",
 ]
 
 
@@ -103,9 +108,32 @@ def load_library(path=LIB_PATH):
     L.ldg_ensemble_step.argtypes = [h, _dp, ctypes.c_int, _dp]
     L.ldg_set_graph.argtypes = [h, ctypes.c_int]
     L.ldg_autotune.argtypes = [h, ip]
+    fp = ctypes.POINTER(ctypes.c_float)
+    lp = ctypes.POINTER(ctypes.c_long)
+    L.ldg_closure_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_long, ctypes.c_ulonglong, ctypes.POINTER(h)]
+    L.ldg_closure_destroy.argtypes = [h]
+    L.ldg_closure_destroy.restype = None
+    L.ldg_closure_last_error.argtypes = [h]
+    L.ldg_closure_last_error.restype = ctypes.c_char_p
+    L.ldg_closure_sizes.argtypes = [h, ip, ip, ip, lp]
+    L.ldg_closure_get.argtypes = [h, fp, fp]
+    L.ldg_closure_set.argtypes = [h, fp, fp]
+    L.ldg_closure_forward.argtypes = [h, fp, ctypes.c_int, fp]
+    L.ldg_closure_gradient.argtypes = [h, fp, fp, ctypes.c_int, fp, _dp]
+    L.ldg_project_state.argtypes = [h, ctypes.c_int, h, ctypes.c_int]
+    L.ldg_closure_collect.argtypes = [h, h, h, ctypes.c_double, ctypes.c_ulonglong]
+    L.ldg_closure_add_samples.argtypes = [h, fp, fp, ctypes.c_long]
+    L.ldg_closure_samples.argtypes = [h, lp]
+    L.ldg_closure_get_samples.argtypes = [h, ctypes.c_long, ctypes.c_long, fp, fp]
+    L.ldg_closure_fit_normalization.argtypes = [h]
+    L.ldg_closure_train.argtypes = [h, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                    ctypes.c_ulonglong, _dp]
+    L.ldg_set_closure.argtypes = [h, h, ctypes.c_double]
+    L.ldg_closure_gated.argtypes = [h, lp, lp]
     for name in EXPORTED:
         fn = getattr(L, name)
-        if name not in ("ldg_destroy", "ldg_last_error", "ldg_time_step"):
+        if name not in ("ldg_destroy", "ldg_last_error", "ldg_time_step", "ldg_closure_destroy",
+                        "ldg_closure_last_error"):
             fn.restype = ctypes.c_int
     _lib = L
     return L

@@ -1,0 +1,196 @@
+"""The learned closure on the GPU (libldg.so ldg_closure_*, through the C ABI).
+
+Checked against the numpy restatement in tests/mlp_numpy.py (itself pinned
+by finite differences and projection identities in test_closure_ref.py):
+- forward pass and parameter gradient, with the BF16 rounding of the
+  matrix-core operands emulated in numpy (FP64 accumulation there, FP32 on
+  the GPU): 1e-3 relative -- an activation that lands within FP32 rounding
+  of a BF16 rounding boundary rounds to the neighbouring BF16 value on one
+  side, one BF16 ulp (2^-8) in a few operands;
+- the element-wise L2 projection between orders: 1e-13 relative;
+- collected training samples against the oracle's right-hand sides of the
+  fine state and of its projection;
+- inside the RK4 stage: a closed gate (bound 0) is bit-identical to the
+  plain solver; an open gate adds exactly the network's correction;
+- training: deterministic in the seed and it reduces the loss on a task
+  the network can represent.
+"""
+import numpy as np
+import pytest
+
+import ldg_amd
+import mlp_numpy as M
+import oracle as O
+from ldg_amd import closure as C
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def test_forward_matches_numpy():
+    c = C.Closure(2, 256, 0, seed=3)
+    params, _ = c.get()
+    rng = np.random.default_rng(0)
+    X = rng.standard_normal((300, c.nin)).astype(np.float32)      # 300: not a multiple of the 64-row tile
+    Y = c.forward(X)
+    Yr, _ = M.forward(params, X, c.nin, c.hidden, c.nout, bf=True)
+    e = rel(Y, Yr)
+    print("forward relative error vs numpy (bf16 operands):", e, flush=True)
+    assert e <= 1e-3
+    # without emulating BF16 the difference is of BF16 size, far above FP32 rounding
+    Yf, _ = M.forward(params, X, c.nin, c.hidden, c.nout, bf=False)
+    assert 1e-5 < rel(Y, Yf) < 5e-2
+
+
+def test_gradient_matches_numpy():
+    c = C.Closure(2, 256, 0, seed=4)
+    params, _ = c.get()
+    rng = np.random.default_rng(1)
+    B = 200
+    X = rng.standard_normal((B, c.nin)).astype(np.float32)
+    T = rng.standard_normal((B, c.nout)).astype(np.float32)
+    g, loss = c.gradient(X, T)
+    gr = M.gradient(params, X, T, c.nin, c.hidden, c.nout, bf=True)
+    Yr, _ = M.forward(params, X, c.nin, c.hidden, c.nout, bf=True)
+    assert abs(loss - np.mean((Yr - T) ** 2)) <= 1e-4 * loss
+    for name, a, b in zip(["W1", "b1", "W2", "b2", "W3", "b3"], c.unpack(g), c.unpack(gr.astype(np.float32))):
+        e = float(np.linalg.norm(a - b) / np.linalg.norm(b))
+        print(f"gradient {name}: relative error {e:.2e}", flush=True)
+        assert e <= 1e-3, name
+
+
+@pytest.mark.parametrize("pf,pc,mesh", [(4, 2, (5, 4, 3)), (5, 2, (3, 3, 3)), (3, 1, (4, 3, 3))])
+def test_projection_matches_numpy(pf, pc, mesh):
+    fine = ldg_amd.LDG(pf, *mesh)
+    coarse = ldg_amd.LDG(pc, *mesh)
+    rng = np.random.default_rng(pf)
+    u = rng.standard_normal(fine.shape)
+    fine.set_state(u)
+    C.project_state(fine, 0, coarse, 0)
+    e = rel(coarse.get_state(), M.project_state(u, pf, pc))
+    assert e <= 1e-13, e
+
+
+def collect_reference(p_f, p_c, ne, u_f):
+    """Features (background removed) and targets from the oracle."""
+    of, oc = O.Oracle(p_f, ne), O.Oracle(p_c, ne)
+    k_f = of.rhs(u_f)
+    Pu = M.project_state(u_f, p_f, p_c)
+    Pk = M.project_state(k_f, p_f, p_c)
+    k_c, du_c = oc.rhs(Pu, want_du=True)
+    K, dofs = Pu.shape[0], Pu.shape[2]
+    gamma, M0, rho0 = 1.4, 0.1, 1.0
+    p0 = rho0 * (1.0 / M0) ** 2 / gamma
+    E0 = p0 / (gamma - 1)
+    w = Pu.copy()
+    w[:, 0] -= rho0
+    w[:, 4] -= E0
+    q = np.transpose(du_c, (1, 0, 2, 3)).reshape(K, 15 * dofs)     # field s + 5 l
+    X = np.concatenate([w.reshape(K, 5 * dofs), q], axis=1)
+    T = (Pk - k_c).reshape(K, 5 * dofs)
+    return X, T
+
+
+def test_collected_samples_match_oracle():
+    ne, pf, pc = 4, 4, 2
+    fine = ldg_amd.Ensemble([{"p": pf, "nx": ne}])
+    coarse = ldg_amd.Ensemble([{"p": pc, "nx": ne}])
+    fine.initial_condition()
+    fine.step(fine.time_steps(0.01), 3, record_ke=False)
+    # break the Taylor-Green symmetries (symmetric elements would have equal features)
+    u_f = fine.get_instance_state(0)
+    u_f[:, 1:4] += 1e-3 * np.random.default_rng(9).standard_normal(u_f[:, 1:4].shape)
+    fine.set_instance_state(0, u_f)
+    c = C.Closure(pc, 64, ne ** 3)
+    c.collect(fine, coarse, keep=1.0, seed=0)
+    assert c.samples == ne ** 3
+    X, T = c.get_samples()
+    Xr, Tr = collect_reference(pf, pc, ne, u_f)
+    ex = np.max(np.abs(X - Xr), axis=0) / np.maximum(np.max(np.abs(Xr), axis=0), 1e-30)
+    print("features max relative error:", ex.max(), " target error:", rel(T, Tr), flush=True)
+    assert ex.max() <= 1e-5
+    assert rel(T, Tr) <= 1e-5
+    # subsampling keeps whole rows of the same set
+    c2 = C.Closure(pc, 64, ne ** 3)
+    c2.collect(fine, coarse, keep=0.25, seed=7)
+    X2, T2 = c2.get_samples()
+    assert len(X2) == 16
+    rows = [int(np.argmin(np.sum((X - x) ** 2, axis=1))) for x in X2]
+    assert np.array_equal(X[rows], X2) and np.array_equal(T[rows], T2)
+
+
+def random_norm(c, rng):
+    nin, nout = c.nin, c.nout
+    return np.concatenate([rng.standard_normal(nin) * 0.1, 0.5 + rng.random(nin),
+                           rng.standard_normal(nout) * 1e-3, 1e-3 * (0.5 + rng.random(nout))]).astype(np.float32)
+
+
+@pytest.mark.parametrize("precision", [ldg_amd.LDG_FP64, ldg_amd.LDG_MIXED])
+def test_closed_gate_is_bitwise_plain(precision):
+    plain = ldg_amd.LDG(2, 4, precision=precision)
+    gated = ldg_amd.LDG(2, 4, precision=precision)
+    c = C.Closure(2, 256, 0, seed=5)
+    c.set(norm=random_norm(c, np.random.default_rng(2)))
+    C.attach(gated, c, 0.0)
+    for g in (plain, gated):
+        g.initial_condition()
+    u = plain.get_state()
+    assert np.array_equal(plain.rhs(u), gated.rhs(u))
+    assert C.gated(gated) == (64, 64)
+    dt = plain.time_step(0.01)
+    ka, kb = plain.step(dt, 3), gated.step(dt, 3)
+    assert np.array_equal(plain.get_state(), gated.get_state())
+    assert np.array_equal(ka, kb)
+
+
+def test_open_gate_adds_the_network_correction():
+    plain = ldg_amd.LDG(2, 4)
+    corr = ldg_amd.LDG(2, 4)
+    c = C.Closure(2, 256, 0, seed=6)
+    rng = np.random.default_rng(3)
+    norm = random_norm(c, rng)
+    c.set(norm=norm)
+    C.attach(corr, c, 1e30)
+    plain.initial_condition()
+    u = plain.get_state()
+    k0, du = plain.rhs(u, want_du=True)
+    k1 = corr.rhs(u)
+    assert C.gated(corr) == (0, 64)
+    # host-side features from the same state and gradient, then the GPU network
+    K, dofs = u.shape[0], u.shape[2]
+    E0 = (1.0 / 0.1) ** 2 / 1.4 / (1.4 - 1.0)
+    w = u.copy()
+    w[:, 0] -= 1.0
+    w[:, 4] -= E0
+    x = np.concatenate([w.reshape(K, -1), np.transpose(du, (1, 0, 2, 3)).reshape(K, -1)], axis=1)
+    nin, nout = c.nin, c.nout
+    X = ((x - norm[:nin]) * norm[nin:2 * nin]).astype(np.float32)
+    Y = c.forward(X).astype(np.float64)
+    expect = (Y * norm[2 * nin + nout:] + norm[2 * nin:2 * nin + nout]).reshape(k0.shape)
+    e = rel(k1 - k0, expect)
+    print("closure correction relative error:", e, flush=True)
+    assert e <= 1e-4
+
+
+def test_training_is_deterministic_and_learns_a_teacher():
+    teacher = C.Closure(2, 256, 0, seed=11)
+    rng = np.random.default_rng(4)
+    n = 8192
+    X = rng.standard_normal((n, teacher.nin)).astype(np.float32)
+    T = teacher.forward(X)
+    runs = []
+    for _ in range(2):
+        s = C.Closure(2, 256, n, seed=12)
+        s.add_samples(X, T)
+        s.fit_normalization()
+        losses = s.train(12, batch=512, lr=1e-3, val_fraction=0.125, seed=5)
+        runs.append((s.get()[0], losses))
+    print("losses (train, val) per epoch:", runs[0][1].tolist(), flush=True)
+    assert np.array_equal(runs[0][0], runs[1][0])
+    losses = runs[0][1]
+    assert np.all(np.isfinite(losses))
+    assert losses[-1, 0] < 0.5 * losses[0, 0]
+    assert losses[-1, 1] < 0.5 * losses[0, 0]
