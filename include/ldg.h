@@ -224,9 +224,12 @@ int ldg_closure_train(ldg_closure *c, int epochs, int batch, double lr, double v
    order: each RK stage (and ldg_rhs) adds the correction element by
    element, unless the RMS of the normalised correction exceeds `bound`
    (that element keeps the plain right-hand side; bound 0 switches every
-   element off).  c must outlive the attachment.  ldg_closure_gated: elements
-   switched off in the last evaluated stage, of `total`. */
-int ldg_set_closure(ldg_handle *h, const ldg_closure *c, double bound);
+   element off) or, with flags & LDG_CLOSURE_DISSIPATIVE, unless the
+   correction would raise the element's nodal kinetic energy.  c must
+   outlive the attachment.  ldg_closure_gated: elements switched off in the
+   last evaluated stage, of `total`. */
+#define LDG_CLOSURE_DISSIPATIVE 1
+int ldg_set_closure(ldg_handle *h, const ldg_closure *c, double bound, int flags);
 int ldg_closure_gated(ldg_handle *h, long *off, long *total);
 
 #ifdef __cplusplus

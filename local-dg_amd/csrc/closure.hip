@@ -32,6 +32,8 @@ struct ldg_closure {
   long nparam = 0;
   float *params = nullptr, *grads = nullptr, *m = nullptr, *v = nullptr;
   float *norm = nullptr;            // in_mean[nin], in_scale[nin] (1/std), out_mean[nout], out_std[nout]
+  unsigned short *Wb = nullptr;     // padded BF16 weight copies for the fused inference kernel
+  int k1p = 0, noutp = 0;
   long adam_t = 0;
   long cap = 0, count = 0;
   float *Xd = nullptr, *Td = nullptr;
@@ -157,6 +159,21 @@ int ensure_workspace(ldg_closure *c, int B) {
   return LDG_OK;
 }
 
+// Refreshes the BF16 weight copies used by the fused inference kernel.
+int pack_bf16(ldg_closure *c) {
+  const Offs o = offsets(c);
+  unsigned short *W1b = c->Wb, *W2b = W1b + (long)c->hidden * c->k1p, *W3b = W2b + (long)c->hidden * c->hidden;
+  const long n = (long)c->hidden * c->k1p + (long)c->hidden * c->hidden + (long)c->noutp * c->hidden;
+  hipLaunchKernelGGL(mlp_pack_bf16, dim3(grid_for(n)), dim3(256), 0, c->stream, (const float *)(c->params + o.W1),
+                     (const float *)(c->params + o.W2), (const float *)(c->params + o.W3), c->nin, c->hidden, c->nout,
+                     c->k1p, c->noutp, W1b, W2b, W3b);
+  CCHECK(c, hipGetLastError());
+  CCHECK(c, hipStreamSynchronize(c->stream));
+  return LDG_OK;
+}
+
+bool fused_ok(const ldg_closure *c) { return c->n == 3 && c->hidden == 256; }
+
 // 1D L2 projection from order pf to order pc on the reference element:
 // P = M_c^{-1} B, B(i, j) = integral of l^c_i l^f_j, both integrals exact
 // (Gauss rules of pc+1 and pf+1 points).
@@ -227,6 +244,17 @@ void features_group(ldg_handle *h, Group &g, const double *U, const float *norm,
   }
 }
 
+template <typename TQ>
+void launch_infer(hipStream_t s, const ldg_closure *c, const double *U, const TQ *Q, long K, double rho0, double E0,
+                  double bound, int dissipative, double *corr, unsigned char *gate) {
+  const Offs o = offsets(c);
+  const unsigned short *W1b = c->Wb, *W2b = W1b + (long)c->hidden * c->k1p, *W3b = W2b + (long)c->hidden * c->hidden;
+  hipLaunchKernelGGL((mlp_infer<TQ, 540, 256, 135>), dim3((unsigned)((K + FR - 1) / FR)), dim3(256), 0, s, U, Q, K,
+                     rho0, E0, (const float *)c->norm, W1b, W2b, W3b, (const float *)(c->params + o.b1),
+                     (const float *)(c->params + o.b2), (const float *)(c->params + o.b3), bound, dissipative,
+                     corr, gate);
+}
+
 void free_closure(ldg_closure *c) {
   float *bufs[] = {c->params, c->grads, c->m, c->v, c->norm, c->Xd, c->Td, c->Xb, c->Tb, c->Z1,
                    c->H1, c->Z2, c->H2, c->Y, c->dY, c->dH, c->dZ};
@@ -234,6 +262,7 @@ void free_closure(ldg_closure *c) {
     if (b) (void)hipFree(b);
   if (c->idx) (void)hipFree(c->idx);
   if (c->part) (void)hipFree(c->part);
+  if (c->Wb) (void)hipFree(c->Wb);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -246,11 +275,31 @@ int closure_stage(ldg_handle *h, Group &g, const double *U, StageArgs &st) {
   GroupClosure &cl = g.cl;
   if (!cl.net) return LDG_OK;
   const ldg_closure *c = cl.net;
-  features_group(h, g, U, c->norm, c->nin, cl.X);
-  forward(c, h->stream, (int)g.K, cl.X, nullptr, cl.H1, nullptr, cl.H2, cl.Y);
-  hipLaunchKernelGGL(mlp_correction, dim3((unsigned)std::min<long>(g.K, 65535)), dim3(64), 0, h->stream,
-                     (const float *)cl.Y, g.K, c->nout, (const float *)(c->norm + 2 * c->nin),
-                     (const float *)(c->norm + 2 * c->nin + c->nout), cl.bound, cl.corr, cl.gate);
+  if (fused_ok(c)) {
+    // one fused launch per instance (each has its own background state)
+    const int dofs = g.n * g.n * g.n;
+    long off = 0;
+    for (int idx : g.members) {
+      const Inst &in = h->inst[idx];
+      const double c0 = in.V0 / in.M0;
+      const double p0 = h->cfg.rho0 * c0 * c0 / h->cfg.gamma;
+      const double E0 = p0 / (h->cfg.gamma - 1.0);
+      if (h->mixed)
+        launch_infer<float>(h->stream, c, U + off * 5 * dofs, (const float *)g.q + off * 15 * dofs, in.K,
+                            h->cfg.rho0, E0, cl.bound, cl.dissipative, cl.corr + off * c->nout, cl.gate + off);
+      else
+        launch_infer<double>(h->stream, c, U + off * 5 * dofs, (const double *)g.q + off * 15 * dofs, in.K,
+                             h->cfg.rho0, E0, cl.bound, cl.dissipative, cl.corr + off * c->nout, cl.gate + off);
+      off += in.K;
+    }
+  } else {
+    features_group(h, g, U, c->norm, c->nin, cl.X);
+    forward(c, h->stream, (int)g.K, cl.X, nullptr, cl.H1, nullptr, cl.H2, cl.Y);
+    hipLaunchKernelGGL(mlp_correction, dim3((unsigned)std::min<long>(g.K, 65535)), dim3(64), 0, h->stream,
+                       (const float *)cl.Y, g.K, c->nout, (const float *)(c->norm + 2 * c->nin),
+                       (const float *)(c->norm + 2 * c->nin + c->nout), cl.bound, cl.dissipative, U, cl.corr,
+                       cl.gate);
+  }
   HIPCHECK(h, hipGetLastError());
   st.corr = cl.corr;
   st.gate = cl.gate;
@@ -289,13 +338,17 @@ int ldg_closure_create(int p, int hidden, long capacity, unsigned long long seed
   const Offs o = offsets(c);
   c->nparam = o.b3 + c->nout;
   c->cap = capacity;
+  c->k1p = (c->nin + 15) / 16 * 16;
+  c->noutp = (c->nout + 31) / 32 * 32;
   auto bad = [&](int code, const char *msg) { g_closure_error = msg; free_closure(c); return code; };
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return bad(LDG_EDEVICE, "hipStreamCreate failed");
   const size_t pb = (size_t)c->nparam * sizeof(float);
   if (hipMalloc((void **)&c->params, pb) != hipSuccess || hipMalloc((void **)&c->grads, pb) != hipSuccess ||
       hipMalloc((void **)&c->m, pb) != hipSuccess || hipMalloc((void **)&c->v, pb) != hipSuccess ||
       hipMalloc((void **)&c->norm, sizeof(float) * 2 * (c->nin + c->nout)) != hipSuccess ||
-      hipMalloc((void **)&c->part, sizeof(double) * kLossBlocks) != hipSuccess)
+      hipMalloc((void **)&c->part, sizeof(double) * kLossBlocks) != hipSuccess ||
+      hipMalloc((void **)&c->Wb, sizeof(unsigned short) * ((size_t)c->hidden * c->k1p + (size_t)c->hidden * c->hidden +
+                                                           (size_t)c->noutp * c->hidden)) != hipSuccess)
     return bad(LDG_ENOMEM, "device allocation failed (closure parameters)");
   if (capacity > 0 && (hipMalloc((void **)&c->Xd, (size_t)capacity * c->nin * sizeof(float)) != hipSuccess ||
                        hipMalloc((void **)&c->Td, (size_t)capacity * c->nout * sizeof(float)) != hipSuccess))
@@ -319,7 +372,7 @@ int ldg_closure_create(int p, int hidden, long capacity, unsigned long long seed
   for (int j = 0; j < c->nout; ++j) { hn[2 * c->nin + j] = 0.0f; hn[2 * c->nin + c->nout + j] = 1.0f; }
   if (hipMemcpy(c->params, hp.data(), pb, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(c->norm, hn.data(), hn.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemset(c->m, 0, pb) != hipSuccess || hipMemset(c->v, 0, pb) != hipSuccess)
+      hipMemset(c->m, 0, pb) != hipSuccess || hipMemset(c->v, 0, pb) != hipSuccess || pack_bf16(c) != LDG_OK)
     return bad(LDG_EDEVICE, "closure initialisation failed");
   *out = c;
   return LDG_OK;
@@ -356,6 +409,8 @@ int ldg_closure_set(ldg_closure *c, const float *params, const float *norm) {
     c->adam_t = 0;
     CCHECK(c, hipMemset(c->m, 0, c->nparam * sizeof(float)));
     CCHECK(c, hipMemset(c->v, 0, c->nparam * sizeof(float)));
+    int rc = pack_bf16(c);
+    if (rc) return rc;
   }
   if (norm) {
     const int nn = 2 * (c->nin + c->nout);
@@ -623,11 +678,11 @@ int ldg_closure_train(ldg_closure *c, int epochs, int batch, double lr, double v
   }
   (void)hipFree(didx);
   if (e != hipSuccess) return cfail(c, LDG_EDEVICE, std::string("training: ") + hipGetErrorString(e));
-  return LDG_OK;
+  return pack_bf16(c);
 }
 
-int ldg_set_closure(ldg_handle *h, const ldg_closure *c, double bound) {
-  if (!h || (c && !(bound >= 0))) return LDG_EINVAL;
+int ldg_set_closure(ldg_handle *h, const ldg_closure *c, double bound, int flags) {
+  if (!h || (c && !(bound >= 0)) || (flags & ~LDG_CLOSURE_DISSIPATIVE)) return LDG_EINVAL;
   HIPCHECK(h, hipStreamSynchronize(h->stream));
   drop_graph(h);
   for (Group &g : h->groups) {
@@ -646,6 +701,7 @@ int ldg_set_closure(ldg_handle *h, const ldg_closure *c, double bound) {
     }
     cl.net = c;
     cl.bound = bound;
+    cl.dissipative = (flags & LDG_CLOSURE_DISSIPATIVE) ? 1 : 0;
   }
   return LDG_OK;
 }

@@ -145,20 +145,40 @@ __global__ void mlp_targets(const double *__restrict__ Pk, const double *__restr
     T[w] = (float)(Pk[w] - k[w]);
 }
 
+// Rate of change of the element's nodal kinetic energy sum_j |m_j|^2 /
+// (2 rho_j) caused by a correction c of (rho, m) (one wave; lanes over the
+// nodes; every lane returns the sum).
+// The correction is c = y * ostd + omean (y: the normalised output row).
+__device__ inline double ke_production(const double *__restrict__ u, int dofs, const float *y,
+                                       const float *__restrict__ ostd, const float *__restrict__ omean, int lane) {
+  auto c = [&](int k) { return (double)y[k] * (double)ostd[k] + (double)omean[k]; };
+  double pr = 0.0;
+  for (int j = lane; j < dofs; j += 64) {
+    const double ir = 1.0 / u[j];
+    const double v0 = u[dofs + j] * ir, v1 = u[2 * dofs + j] * ir, v2 = u[3 * dofs + j] * ir;
+    pr += v0 * c(dofs + j) + v1 * c(2 * dofs + j) + v2 * c(3 * dofs + j) - 0.5 * (v0 * v0 + v1 * v1 + v2 * v2) * c(j);
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) pr += __shfl_xor(pr, o, 64);
+  return pr;
+}
+
 // Closure correction of each element (one 64-thread block per element):
-// corr = Y * ostd + omean, applied only if the RMS of the normalised
-// output Y is at most `bound` (gate = 1), else gate = 0 (plain RHS).
+// corr = Y * ostd + omean, applied (gate = 1) only if the RMS of the
+// normalised output Y is at most `bound` and, with `dissipative`, if it
+// does not raise the element's kinetic energy; else gate = 0 (plain RHS).
 __global__ void __launch_bounds__(64) mlp_correction(const float *__restrict__ Y, long K, int nout,
                                                      const float *__restrict__ omean, const float *__restrict__ ostd,
-                                                     double bound, double *__restrict__ corr,
-                                                     unsigned char *__restrict__ gate) {
+                                                     double bound, int dissipative, const double *__restrict__ U,
+                                                     double *__restrict__ corr, unsigned char *__restrict__ gate) {
   for (long e = blockIdx.x; e < K; e += gridDim.x) {
     const float *y = Y + e * nout;
     float ss = 0.0f;
     for (int j = threadIdx.x; j < nout; j += 64) ss += y[j] * y[j];
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) ss += __shfl_xor(ss, o, 64);
-    const bool on = sqrt((double)ss / nout) <= bound;
+    bool on = sqrt((double)ss / nout) <= bound;
+    if (on && dissipative) on = ke_production(U + e * nout, nout / 5, y, ostd, omean, threadIdx.x) <= 0.0;
     for (int j = threadIdx.x; j < nout; j += 64)
       corr[e * nout + j] = on ? (double)y[j] * (double)ostd[j] + (double)omean[j] : 0.0;
     if (threadIdx.x == 0) gate[e] = on ? 1 : 0;
@@ -253,6 +273,200 @@ __global__ void mlp_normalize(float *__restrict__ D, long rows, int width, const
   for (long w = blockIdx.x * (long)blockDim.x + threadIdx.x; w < total; w += (long)gridDim.x * blockDim.x) {
     const int j = (int)(w % width);
     D[w] = (D[w] - mean[j]) * scale[j];
+  }
+}
+
+// BF16 copies of the weight matrices for the fused inference kernel, rows
+// padded to a multiple of 16 columns (W1) and the output matrix to a
+// multiple of 32 rows (W3), padding zero:
+//   W1b [hid][k1p], W2b [hid][hid], W3b [noutp][hid].
+__global__ void mlp_pack_bf16(const float *__restrict__ W1, const float *__restrict__ W2,
+                              const float *__restrict__ W3, int nin, int hid, int nout, int k1p, int noutp,
+                              unsigned short *__restrict__ W1b, unsigned short *__restrict__ W2b,
+                              unsigned short *__restrict__ W3b) {
+  const long n1 = (long)hid * k1p, n2 = (long)hid * hid, n3 = (long)noutp * hid;
+  for (long w = blockIdx.x * (long)blockDim.x + threadIdx.x; w < n1 + n2 + n3; w += (long)gridDim.x * blockDim.x) {
+    if (w < n1) {
+      const long r = w / k1p;
+      const int c = (int)(w - r * k1p);
+      W1b[w] = c < nin ? to_bf16(W1[r * nin + c]) : 0;
+    } else if (w < n1 + n2) {
+      W2b[w - n1] = to_bf16(W2[w - n1]);
+    } else {
+      const long v = w - n1 - n2;
+      const long r = v / hid;
+      W3b[v] = r < nout ? to_bf16(W3[v]) : 0;
+    }
+  }
+}
+
+// Fused inference for p = 2 elements (27 nodes: NIN = 540 inputs, NOUT =
+// 135 outputs) and HID = 256: one workgroup evaluates the network on FR = 32
+// elements of one instance entirely on chip -- features from the state and
+// gradient (background removed, normalised, rounded to BF16) into LDS, three
+// MFMA layers with hidden activations kept in LDS, then the gate and the
+// correction.  Numerically identical to mlp_features + 3 x mlp_gemm +
+// mlp_correction (same roundings, same MFMA sequence over k).
+constexpr int FR = 32;
+
+// One wave: acc[t] += A (32 rows from LDS, row stride LDA) x W_t^T over KS
+// steps of 16, with W_t the 32 weight rows starting at w[t] (BF16, global).
+// Weight vectors are prefetched PF steps ahead so that the L2 latency is
+// covered by the matrix-core work of the steps in between.
+template <int KS, int NTILE, int LDA>
+__device__ inline void wave_layer(const unsigned short *__restrict__ A, const unsigned short *const (&w)[NTILE],
+                                  int lr, int kh, floatx16 (&acc)[NTILE]) {
+  constexpr int PF = 4;
+  u16x8 buf[PF][NTILE];
+#pragma unroll
+  for (int p = 0; p < PF; ++p)
+#pragma unroll
+    for (int t = 0; t < NTILE; ++t)
+      if (p < KS) buf[p][t] = *reinterpret_cast<const u16x8 *>(w[t] + 16 * p + kh);
+#pragma unroll
+  for (int s = 0; s < KS; ++s) {
+    u16x8 cur[NTILE];
+#pragma unroll
+    for (int t = 0; t < NTILE; ++t) {
+      cur[t] = buf[s % PF][t];
+      if (s + PF < KS) buf[s % PF][t] = *reinterpret_cast<const u16x8 *>(w[t] + 16 * (s + PF) + kh);
+    }
+    const u16x8 av = *reinterpret_cast<const u16x8 *>(A + lr * LDA + 16 * s + kh);
+#pragma unroll
+    for (int t = 0; t < NTILE; ++t)
+      acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, av), __builtin_bit_cast(bf16x8, cur[t]),
+                                                       acc[t], 0, 0, 0);
+  }
+}
+
+template <typename TQ, int NIN, int HID, int NOUT>
+__global__ void __launch_bounds__(256) mlp_infer(const double *__restrict__ U, const TQ *__restrict__ Q, long K,
+                                                 double rho0, double E0, const float *__restrict__ norm,
+                                                 const unsigned short *__restrict__ W1b,
+                                                 const unsigned short *__restrict__ W2b,
+                                                 const unsigned short *__restrict__ W3b, const float *__restrict__ b1,
+                                                 const float *__restrict__ b2, const float *__restrict__ b3,
+                                                 double bound, int dissipative, double *__restrict__ corr,
+                                                 unsigned char *__restrict__ gate) {
+  constexpr int DOFS = NIN / 20, NU = 5 * DOFS, NQ = 15 * DOFS;
+  constexpr int K1P = (NIN + 15) / 16 * 16, NOUTP = (NOUT + 31) / 32 * 32;
+  constexpr int XS = K1P + 8, HS = HID + 8;
+  static_assert(HID == 256, "four waves of 64 hidden columns");
+  static_assert(XS >= HS, "the second hidden layer reuses the feature buffer");
+  __shared__ __align__(16) unsigned short Xs[FR][XS];
+  __shared__ __align__(16) unsigned short Hs[FR][HS];
+  __shared__ float Ys[FR][NOUT + 1];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const long e0 = (long)blockIdx.x * FR;
+  const int rows = (int)((K - e0) < FR ? (K - e0) : FR);
+  const float *mean = norm, *scale = norm + NIN, *omean = norm + 2 * NIN, *ostd = norm + 2 * NIN + NOUT;
+
+  // normalisation to LDS, then the block's state and gradient rows, which
+  // are contiguous in HBM: each thread issues a batch of loads before using
+  // any of them
+  __shared__ float smean[NIN], sscale[NIN];
+  for (int j = tid; j < NIN; j += 256) { smean[j] = mean[j]; sscale[j] = scale[j]; }
+  for (int w = tid; w < FR * (K1P - NIN); w += 256) Xs[w / (K1P - NIN)][NIN + w % (K1P - NIN)] = 0;
+  __syncthreads();
+  {
+    constexpr int B = 8;
+    const double *Ub = U + e0 * NU;
+    for (int base = 0; base < FR * NU; base += 256 * B) {
+      double v[B];
+#pragma unroll
+      for (int i = 0; i < B; ++i) {
+        const int idx = base + i * 256 + tid;
+        v[i] = idx < rows * NU ? Ub[idx] : 0.0;
+      }
+#pragma unroll
+      for (int i = 0; i < B; ++i) {
+        const int idx = base + i * 256 + tid;
+        if (idx < FR * NU) {
+          const int r = idx / NU, j = idx - r * NU, s = j / DOFS;
+          const double x = v[i] - (s == 0 ? rho0 : s == 4 ? E0 : 0.0);
+          Xs[r][j] = to_bf16(r < rows ? (float)((x - (double)smean[j]) * (double)sscale[j]) : 0.0f);
+        }
+      }
+    }
+    const TQ *Qb = Q + e0 * NQ;
+    for (int base = 0; base < FR * NQ; base += 256 * B) {
+      double v[B];
+#pragma unroll
+      for (int i = 0; i < B; ++i) {
+        const int idx = base + i * 256 + tid;
+        v[i] = idx < rows * NQ ? (double)Qb[idx] : 0.0;
+      }
+#pragma unroll
+      for (int i = 0; i < B; ++i) {
+        const int idx = base + i * 256 + tid;
+        if (idx < FR * NQ) {
+          const int r = idx / NQ, j = NU + (idx - r * NQ);
+          Xs[r][j] = to_bf16(r < rows ? (float)((v[i] - (double)smean[j]) * (double)sscale[j]) : 0.0f);
+        }
+      }
+    }
+  }
+  __syncthreads();
+
+  const int lr = lane & 31, kh = (lane >> 5) * 8;
+  // layer 1: wave w owns hidden columns [64 w, 64 w + 64)
+  {
+    floatx16 a[2];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) { a[0][j] = 0.0f; a[1][j] = 0.0f; }
+    const unsigned short *const w[2] = {W1b + (long)(64 * wave + lr) * K1P, W1b + (long)(64 * wave + 32 + lr) * K1P};
+    wave_layer<K1P / 16, 2, XS>(&Xs[0][0], w, lr, kh, a);
+    const int c0 = 64 * wave + lr;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      const int r = 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
+      Hs[r][c0] = to_bf16(silu(a[0][j] + b1[c0]));
+      Hs[r][c0 + 32] = to_bf16(silu(a[1][j] + b1[c0 + 32]));
+    }
+  }
+  __syncthreads();
+  // layer 2 into the feature buffer
+  {
+    floatx16 a[2];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) { a[0][j] = 0.0f; a[1][j] = 0.0f; }
+    const unsigned short *const w[2] = {W2b + (long)(64 * wave + lr) * HID, W2b + (long)(64 * wave + 32 + lr) * HID};
+    wave_layer<HID / 16, 2, HS>(&Hs[0][0], w, lr, kh, a);
+    const int c0 = 64 * wave + lr;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      const int r = 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
+      Xs[r][c0] = to_bf16(silu(a[0][j] + b2[c0]));
+      Xs[r][c0 + 32] = to_bf16(silu(a[1][j] + b2[c0 + 32]));
+    }
+  }
+  __syncthreads();
+  // layer 3: output tiles of 32 columns, tile t on wave t mod 4
+  for (int t = wave; t < NOUTP / 32; t += 4) {
+    floatx16 a[1];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) a[0][j] = 0.0f;
+    const unsigned short *const w[1] = {W3b + (long)(32 * t + lr) * HID};
+    wave_layer<HID / 16, 1, XS>(&Xs[0][0], w, lr, kh, a);
+    const int c = 32 * t + lr;
+    if (c < NOUT) {
+#pragma unroll
+      for (int j = 0; j < 16; ++j) Ys[8 * (j >> 2) + 4 * (lane >> 5) + (j & 3)][c] = a[0][j] + b3[c];
+    }
+  }
+  __syncthreads();
+  // gate and correction, one wave per row
+  for (int r = wave; r < rows; r += 4) {
+    float ss = 0.0f;
+    for (int j = lane; j < NOUT; j += 64) ss += Ys[r][j] * Ys[r][j];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) ss += __shfl_xor(ss, o, 64);
+    bool on = sqrt((double)ss / NOUT) <= bound;
+    const long e = e0 + r;
+    if (on && dissipative) on = ke_production(U + e * NU, DOFS, &Ys[r][0], ostd, omean, lane) <= 0.0;
+    for (int j = lane; j < NOUT; j += 64)
+      corr[e * NOUT + j] = on ? (double)Ys[r][j] * (double)ostd[j] + (double)omean[j] : 0.0;
+    if (lane == 0) gate[e] = on ? 1 : 0;
   }
 }
 

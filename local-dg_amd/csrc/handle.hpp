@@ -41,6 +41,7 @@ struct Tiling {
 struct GroupClosure {
   const ldg_closure *net = nullptr;
   double bound = 0.0;                 // gate: correction RMS in units of the training spread
+  int dissipative = 0;                // gate: also switch off corrections that raise the element's KE
   float *X = nullptr, *H1 = nullptr, *H2 = nullptr, *Y = nullptr;
   double *corr = nullptr;             // per element, state layout
   unsigned char *gate = nullptr;      // per element: 1 = correction applied

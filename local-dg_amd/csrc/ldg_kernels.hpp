@@ -296,22 +296,60 @@ __global__ void __launch_bounds__(NT) ldg_grad_kernel(const double *__restrict__
     const int i = tid / (N + 1), j = tid % (N + 1);
     sD[i][j] = j < N ? opp->Dn[i][j] : opp->L0[i];
   }
-  const double *Ub = U + k0 * NSTATE * NN;
-  for (int w = tid; w < ne * NSTATE * NN; w += NT) (&su[0][0][0])[w] = Ub[w];
+  // the block's state (contiguous) into LDS: all of a thread's loads are
+  // issued before the first is stored
+  {
+    constexpr int UL = (E * NSTATE * NN + NT - 1) / NT;
+    const double *Ub = U + k0 * NSTATE * NN;
+    const int tot = ne * NSTATE * NN;
+    double v[UL];
+#pragma unroll
+    for (int it = 0; it < UL; ++it) {
+      const int w = tid + it * NT;
+      v[it] = w < tot ? Ub[w] : 0.0;
+    }
+#pragma unroll
+    for (int it = 0; it < UL; ++it) {
+      const int w = tid + it * NT;
+      if (w < tot) (&su[0][0][0])[w] = v[it];
+    }
+  }
   __syncthreads();
 
   // jump on the negative face of direction d: own trace minus the -d
-  // neighbour's trace on its positive face
-  for (int w = tid; w < ne * 3 * NSTATE * NF; w += NT) {
-    const int e = w / (3 * NSTATE * NF);
-    const int r = w - e * (3 * NSTATE * NF);
-    const int d = r / (NSTATE * NF);
-    const int s = (r / NF) % NSTATE;
-    const int ab = r % NF;
-    const int a = ab % N, b = ab / N;
-    const long Kn = snb[e][2 * d];
-    const int base = line_base<N>(d, a, b), st = line_stride<N>(d);
-    sj[e][d][s][ab] = su[e][s][base] - U[(Kn * NSTATE + s) * NN + base + (N - 1) * st];
+  // neighbour's trace on its positive face (neighbour loads batched too)
+  {
+    constexpr int JL = (E * 3 * NSTATE * NF + NT - 1) / NT;
+    const int tot = ne * 3 * NSTATE * NF;
+    double v[JL];
+#pragma unroll
+    for (int it = 0; it < JL; ++it) {
+      const int w = tid + it * NT;
+      v[it] = 0.0;
+      if (w < tot) {
+        const int e = w / (3 * NSTATE * NF);
+        const int r = w - e * (3 * NSTATE * NF);
+        const int d = r / (NSTATE * NF);
+        const int s = (r / NF) % NSTATE;
+        const int ab = r % NF;
+        const long Kn = snb[e][2 * d];
+        const int base = line_base<N>(d, ab % N, ab / N), st = line_stride<N>(d);
+        v[it] = U[(Kn * NSTATE + s) * NN + base + (N - 1) * st];
+      }
+    }
+#pragma unroll
+    for (int it = 0; it < JL; ++it) {
+      const int w = tid + it * NT;
+      if (w < tot) {
+        const int e = w / (3 * NSTATE * NF);
+        const int r = w - e * (3 * NSTATE * NF);
+        const int d = r / (NSTATE * NF);
+        const int s = (r / NF) % NSTATE;
+        const int ab = r % NF;
+        const int base = line_base<N>(d, ab % N, ab / N);
+        sj[e][d][s][ab] = su[e][s][base] - v[it];
+      }
+    }
   }
   __syncthreads();
 
@@ -916,8 +954,20 @@ static __global__ void __launch_bounds__(NT) ldg_finalize_kernel(const double *_
   const int b0 = inst[i].block_off, b1 = b0 + inst[i].nblocks;
   const long row = step ? *step : 0;
   for (int j = 0; j < stride; ++j) {
-    double v = 0.0;
-    for (int b = b0 + threadIdx.x; b < b1; b += NT) v += part[(long)b * stride + j];
+    // thread t sums blocks b0 + t + k NT into accumulator k mod 8 (eight
+    // loads in flight; the order depends only on b - b0)
+    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int base = b0 + threadIdx.x; base < b1; base += 8 * NT) {
+      double x[8];
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        const int b = base + i * NT;
+        x[i] = b < b1 ? part[(long)b * stride + j] : 0.0;
+      }
+#pragma unroll
+      for (int i = 0; i < 8; ++i) acc[i] += x[i];
+    }
+    const double v = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
     const double t = block_sum(v, sred);
     if (threadIdx.x == 0) out[(row * row_stride + i) * stride + j] = t * scale[i];
     __syncthreads();

@@ -25,6 +25,7 @@ LIB_PATH = os.path.join(_HERE, "libldg.so")
 
 LDG_OK, LDG_EINVAL, LDG_ENOMEM, LDG_EDEVICE, LDG_ENONFINITE = 0, -1, -2, -3, -4
 LDG_FP64, LDG_MIXED = 0, 1
+LDG_CLOSURE_DISSIPATIVE = 1
 
 EXPORTED = [
     "ldg_config_default", "ldg_create", "ldg_destroy", "ldg_last_error", "ldg_sizes",
@@ -128,7 +129,7 @@ def load_library(path=LIB_PATH):
     L.ldg_closure_fit_normalization.argtypes = [h]
     L.ldg_closure_train.argtypes = [h, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                     ctypes.c_ulonglong, _dp]
-    L.ldg_set_closure.argtypes = [h, h, ctypes.c_double]
+    L.ldg_set_closure.argtypes = [h, h, ctypes.c_double, ctypes.c_int]
     L.ldg_closure_gated.argtypes = [h, lp, lp]
     for name in EXPORTED:
         fn = getattr(L, name)

@@ -24,7 +24,7 @@ import time
 
 import numpy as np
 
-from . import LDG, LDG_FP64, LDG_MIXED, LDG_OK, Ensemble, LDGError, load_library
+from . import LDG, LDG_CLOSURE_DISSIPATIVE, LDG_FP64, LDG_MIXED, LDG_OK, Ensemble, LDGError, load_library
 
 _fp = ctypes.POINTER(ctypes.c_float)
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -156,10 +156,12 @@ def project_state(fine, i, coarse, j):
         raise LDGError(rc, lib.ldg_last_error(coarse._h).decode())
 
 
-def attach(solver, closure, bound):
-    """ldg_set_closure: closure=None detaches."""
+def attach(solver, closure, bound, dissipative=False):
+    """ldg_set_closure: closure=None detaches; dissipative=True also gates
+    off corrections that would raise an element's kinetic energy."""
     lib = load_library()
-    rc = lib.ldg_set_closure(solver._h, closure._c if closure is not None else None, bound)
+    rc = lib.ldg_set_closure(solver._h, closure._c if closure is not None else None, bound,
+                             LDG_CLOSURE_DISSIPATIVE if dissipative else 0)
     if rc != LDG_OK:
         raise LDGError(rc, lib.ldg_last_error(solver._h).decode())
 
@@ -219,7 +221,7 @@ def generate(closure, runs=4, p_fine=5, ne=24, t_end=10.0, dt_snap=0.1, keep=0.2
         closure.collect(fine, coarse, keep, seed * 100003 + k)
         if k == nsnap:
             break
-        fine.step(dt_vec, n_per_snap, record_ke=False)
+        fine.step(dt_vec, n_per_snap, record_ke=True)     # raises if a run blows up
         if log and (k % 10 == 9 or k == 0):
             log(f"closure data: snapshot {k + 1}/{nsnap} t={(k + 1) * dt_snap:.2f} "
                 f"samples={closure.samples} wall={time.perf_counter() - t0:.1f}s")
@@ -231,14 +233,22 @@ def generate(closure, runs=4, p_fine=5, ne=24, t_end=10.0, dt_snap=0.1, keep=0.2
 
 
 def run_tgv(p, ne, t_end, closure=None, bound=5.0, precision=LDG_FP64, cfl=0.01, dt_diag=0.1, Re=1600.0,
-            M0=0.1, log=None):
+            M0=0.1, log=None, dissipative=False, ic_from=None):
     """Taylor-Green at order p on ne^3 to t_end (reference time-step rule);
     returns times and reference KE after every step, Gauss-point KE and
-    enstrophy every dt_diag, wall time of the stepping, gated elements."""
+    enstrophy every dt_diag, wall time of the stepping, gated elements.
+    ic_from = q > p starts from the L2 projection of the order-q initial
+    state (the states the closure is trained on) instead of the nodal one."""
     g = LDG(p, ne, Re=Re, M0=M0, precision=precision)
     if closure is not None:
-        attach(g, closure, bound)
-    g.initial_condition()
+        attach(g, closure, bound, dissipative)
+    if ic_from:
+        f = LDG(ic_from, ne, Re=Re, M0=M0)
+        f.initial_condition()
+        project_state(f, 0, g, 0)
+        f.close()
+    else:
+        g.initial_condition()
     n_per, dt = steps_to(g.time_step(cfl), dt_diag)
     ndiag = int(round(t_end / dt_diag))
     kes, diag = [], [g.diagnostics()]
@@ -274,6 +284,41 @@ def compare(run, ref):
     return float(np.max(np.abs(d_run - d_ref)) / np.max(np.abs(d_ref)))
 
 
+def apriori(closure, times, ne=24, p_fine=5, Re=1600.0, M0=0.1, seed=12345, precision=LDG_MIXED, cfl=0.01,
+            log=None):
+    """A-priori accuracy of a trained closure on a fresh fine run: at each
+    time, the relative error ||c - T|| / ||T|| of the correction c against
+    the true target T = P rhs_fine - rhs_coarse(P u_fine), and the
+    explained fraction 1 - ||c - T||^2 / ||T - mean T||^2."""
+    inst = [{"nx": ne, "Re": Re, "M0": M0, "V0": 1.0, "phase": (0.0, 0.0, 0.0)}]
+    fine = Ensemble([dict(inst[0], p=p_fine)], precision=precision)
+    coarse = Ensemble([dict(inst[0], p=closure.p)])
+    fine.initial_condition()
+    _, norm = closure.get()
+    nin, nout = closure.nin, closure.nout
+    out, t_now = [], 0.0
+    for t in times:
+        if t > t_now:
+            n, dt = steps_to(fine.time_steps(cfl)[0], t - t_now)
+            fine.step(np.array([dt]), n, record_ke=True)
+            t_now = t
+        probe = Closure(closure.p, 16, ne ** 3)
+        probe.collect(fine, coarse, keep=1.0, seed=seed)
+        X, T = probe.get_samples()
+        probe.close()
+        Y = closure.forward(((X - norm[:nin]) * norm[nin:2 * nin]).astype(np.float32)).astype(np.float64)
+        c = Y * norm[2 * nin + nout:] + norm[2 * nin:2 * nin + nout]
+        T = T.astype(np.float64)
+        rel_err = float(np.linalg.norm(c - T) / np.linalg.norm(T))
+        expl = float(1.0 - np.sum((c - T) ** 2) / np.sum((T - T.mean(0)) ** 2))
+        out.append({"t": t, "rel_err": rel_err, "explained": expl, "target_rms": float(np.sqrt(np.mean(T ** 2)))})
+        if log:
+            log(f"a priori t={t}: relative error {rel_err:.3f}, explained {expl:.3f}")
+    fine.close()
+    coarse.close()
+    return out
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(prog="python -m ldg_amd.closure")
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -294,12 +339,25 @@ def main(argv=None):
     ev.add_argument("--weights", required=True)
     ev.add_argument("--ne", type=int, default=24)
     ev.add_argument("--t-end", type=float, default=10.0)
-    ev.add_argument("--bound", type=float, default=5.0)
+    ev.add_argument("--bounds", default="5", help="comma-separated gate bounds")
+    ev.add_argument("--gates", default="plain", help="comma-separated: plain, dissipative")
+    ev.add_argument("--relax", default="1", help="comma-separated factors multiplying the correction")
+    ev.add_argument("--ic-from", type=int, default=0, help="start p=2 runs from the projected order-q state")
     ev.add_argument("--ref-p", type=int, default=5)
+    ev.add_argument("--ref-json", default=None, help="take the resolved run from an earlier evaluate output")
     ev.add_argument("--no-ref", action="store_true")
     ev.add_argument("--out", default=None)
+    ap_ = sub.add_parser("apriori")
+    ap_.add_argument("--weights", required=True)
+    ap_.add_argument("--ne", type=int, default=24)
+    ap_.add_argument("--times", default="0,1,2,4,6,8,9.5")
     a = ap.parse_args(argv)
     log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
+    if a.cmd == "apriori":
+        c = Closure.load(a.weights)
+        res = apriori(c, [float(t) for t in a.times.split(",")], a.ne, log=log)
+        print(json.dumps(res), flush=True)
+        return
     if a.cmd == "train":
         ne, t_end = a.ne, a.t_end
         nsnap = int(round(t_end / a.dt_snap)) + 1
@@ -315,16 +373,32 @@ def main(argv=None):
         print(json.dumps({k: v for k, v in info.items() if k != "instances"}), flush=True)
     else:
         c = Closure.load(a.weights)
-        res = {"bound": a.bound}
-        runs = {}
-        runs["closure"] = run_tgv(2, a.ne, a.t_end, c, a.bound, log=log)
-        runs["plain_p2"] = run_tgv(2, a.ne, a.t_end, log=log)
-        if not a.no_ref:
-            runs[f"plain_p{a.ref_p}"] = run_tgv(a.ref_p, a.ne, a.t_end, log=log)
-            ref = runs[f"plain_p{a.ref_p}"]
-            res["dEk_dt_err_closure"] = compare(runs["closure"], ref)
-            res["dEk_dt_err_plain_p2"] = compare(runs["plain_p2"], ref)
-            res["wall_ratio_ref_over_closure"] = ref["wall_s"] / runs["closure"]["wall_s"]
+        res, runs = {}, {}
+        params, norm = c.get()
+        nin, nout = c.nin, c.nout
+        for relax in [float(r) for r in a.relax.split(",")]:
+            n2 = norm.copy()
+            n2[2 * nin:] *= relax          # correction = relax * (y * std + mean); the gate sees y
+            c.set(norm=n2)
+            for bound in [float(b) for b in a.bounds.split(",")]:
+                for gate in a.gates.split(","):
+                    runs[f"closure_b{bound:g}_{gate}_r{relax:g}"] = run_tgv(
+                        2, a.ne, a.t_end, c, bound, log=log, dissipative=(gate == "dissipative"), ic_from=a.ic_from)
+        c.set(norm=norm)
+        runs["plain_p2"] = run_tgv(2, a.ne, a.t_end, log=log, ic_from=a.ic_from)
+        ref = None
+        if a.ref_json:
+            d = json.load(open(a.ref_json))[f"plain_p{a.ref_p}"]
+            ref = {"t_diag": np.array(d["t"]), "ke_gauss": np.array(d["ke_gauss"]), "wall_s": d["wall_s"],
+                   "steps": d["steps"], "gated_max": 0, "enstrophy": np.array(d["enstrophy"])}
+        elif not a.no_ref:
+            ref = run_tgv(a.ref_p, a.ne, a.t_end, log=log)
+        if ref is not None:
+            runs[f"plain_p{a.ref_p}"] = ref
+            for k, r in runs.items():
+                if k != f"plain_p{a.ref_p}":
+                    res[f"dEk_dt_err_{k}"] = compare(r, ref)
+                    res[f"wall_ratio_ref_over_{k}"] = ref["wall_s"] / r["wall_s"]
         for k, r in runs.items():
             res[k] = {"wall_s": r["wall_s"], "steps": r["steps"], "gated_max": r["gated_max"],
                       "t": r["t_diag"].tolist(), "ke_gauss": r["ke_gauss"].tolist(),

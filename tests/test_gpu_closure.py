@@ -146,10 +146,14 @@ def test_closed_gate_is_bitwise_plain(precision):
     assert np.array_equal(ka, kb)
 
 
-def test_open_gate_adds_the_network_correction():
+@pytest.mark.parametrize("hidden", [256, 64])
+def test_open_gate_adds_the_network_correction(hidden):
+    """hidden 256 at p = 2 runs the fused on-chip inference kernel in the
+    stage, other widths the layer-by-layer path; both must add what the
+    network computes on the same features."""
     plain = ldg_amd.LDG(2, 4)
     corr = ldg_amd.LDG(2, 4)
-    c = C.Closure(2, 256, 0, seed=6)
+    c = C.Closure(2, hidden, 0, seed=6)
     rng = np.random.default_rng(3)
     norm = random_norm(c, rng)
     c.set(norm=norm)
@@ -194,3 +198,38 @@ def test_training_is_deterministic_and_learns_a_teacher():
     assert np.all(np.isfinite(losses))
     assert losses[-1, 0] < 0.5 * losses[0, 0]
     assert losses[-1, 1] < 0.5 * losses[0, 0]
+
+
+@pytest.mark.parametrize("hidden", [256, 64])
+def test_dissipative_gate_keeps_only_energy_removing_corrections(hidden):
+    """With the dissipative gate an element keeps its correction only if
+    the correction does not raise its nodal kinetic energy
+    sum_j |m_j|^2 / (2 rho_j); the others keep the plain right-hand side."""
+    plain = ldg_amd.LDG(2, 4)
+    corr = ldg_amd.LDG(2, 4)
+    c = C.Closure(2, hidden, 0, seed=8)
+    rng = np.random.default_rng(5)
+    norm = random_norm(c, rng)
+    c.set(norm=norm)
+    C.attach(corr, c, 1e30, dissipative=True)
+    plain.initial_condition()
+    u = plain.get_state()
+    k0, du = plain.rhs(u, want_du=True)
+    k1 = corr.rhs(u)
+    K, dofs = u.shape[0], u.shape[2]
+    E0 = (1.0 / 0.1) ** 2 / 1.4 / (1.4 - 1.0)
+    w = u.copy()
+    w[:, 0] -= 1.0
+    w[:, 4] -= E0
+    x = np.concatenate([w.reshape(K, -1), np.transpose(du, (1, 0, 2, 3)).reshape(K, -1)], axis=1)
+    nin, nout = c.nin, c.nout
+    Y = c.forward(((x - norm[:nin]) * norm[nin:2 * nin]).astype(np.float32)).astype(np.float64)
+    full = (Y * norm[2 * nin + nout:] + norm[2 * nin:2 * nin + nout]).reshape(k0.shape)
+    v = u[:, 1:4] / u[:, 0:1]
+    prod = np.sum(np.sum(v * full[:, 1:4], axis=1) - 0.5 * np.sum(v * v, axis=1) * full[:, 0], axis=1)
+    on = prod <= 0
+    off, total = C.gated(corr)
+    assert total == K and off == int(np.sum(~on)) and 0 < off < K
+    d = k1 - k0
+    assert np.max(np.abs(d[~on])) == 0.0
+    assert rel(d[on], full[on]) <= 1e-4
