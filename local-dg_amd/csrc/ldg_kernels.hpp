@@ -274,8 +274,20 @@ __device__ inline double block_sum(double v, double *sred) {
 // components and stores them straight to HBM (consecutive threads ->
 // consecutive nodes, coalesced).
 // ---------------------------------------------------------------------
+// Occupancy target of the gradient kernel (waves per SIMD): 6 with an FP32
+// gradient (measured 0.79 vs 0.87 ms at 64^3, p = 4), 5 (the compiler's own
+// choice) with FP64; never more than its LDS footprint allows (one
+// workgroup = one wave per SIMD, 160 KiB of LDS per CU).
 template <int N, int E, typename TQ>
-__global__ void __launch_bounds__(NT) ldg_grad_kernel(const double *__restrict__ U, TQ *__restrict__ Q,
+struct GradWaves {
+  static constexpr int lds = 8 * (E * NSTATE * N * N * N + E * 3 * NSTATE * N * N + N * (N + 1) + 6 * E);
+  static constexpr int by_lds = (160 * 1024) / lds;
+  static constexpr int want = sizeof(TQ) == 4 ? 6 : 5;
+  static constexpr int value = want < by_lds ? want : by_lds;
+};
+
+template <int N, int E, typename TQ>
+__global__ void __launch_bounds__(NT, (GradWaves<N, E, TQ>::value)) ldg_grad_kernel(const double *__restrict__ U, TQ *__restrict__ Q,
                                                         LaunchArgs la, const Ops<N, double> *__restrict__ opp) {
   constexpr int NN = N * N * N, NF = N * N;
   __shared__ double su[E][NSTATE][NN];
