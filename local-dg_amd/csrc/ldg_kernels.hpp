@@ -150,6 +150,14 @@ __device__ inline int line_stride(int d) { return d == 0 ? 1 : d == 1 ? N : N * 
 // E - E0); p' = p - p0 = (gamma-1)(w4 - |m|^2/(2 rho)) and
 // E + p = H0 + w4 + p'.
 // ---------------------------------------------------------------------
+// Reciprocal and square root: correctly rounded in FP64; in FP32 (the
+// mixed path, whose error budget is set by FP32 rounding anyway) the
+// hardware approximations v_rcp_f32 / v_sqrt_f32 (1 ulp).
+__device__ inline double rcp(double x) { return 1.0 / x; }
+__device__ inline float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ inline double sqrt_(double x) { return sqrt(x); }
+__device__ inline float sqrt_(float x) { return __builtin_amdgcn_sqrtf(x); }
+
 template <typename T>
 struct Bg {
   T rho0, E0, H0, gm1, mu, kappa;
@@ -179,7 +187,7 @@ __device__ inline void prim_grads(T ir, T E, const T v[3], const T *g, T gv[3][3
 template <typename T>
 __device__ inline void volume_flux(const T w[5], const T *g, const Bg<T> &b, T F[15]) {
   const T rho = b.rho0 + w[0];
-  const T ir = T(1) / rho;
+  const T ir = rcp(rho);
   const T v[3] = {w[1] * ir, w[2] * ir, w[3] * ir};
   const T pf = b.gm1 * (w[4] - T(0.5) * (w[1] * v[0] + w[2] * v[1] + w[3] * v[2]));
   const T hv = b.H0 + w[4] + pf;   // E + p
@@ -205,7 +213,7 @@ __device__ inline void volume_flux(const T w[5], const T *g, const Bg<T> &b, T F
 template <typename T, int D>
 __device__ inline void face_flux(const T wK[5], const T wN[5], const T *q, T sgn, const Bg<T> &b, T out[5]) {
   const T rN = b.rho0 + wN[0], rK = b.rho0 + wK[0];
-  const T irN = T(1) / rN, irK = T(1) / rK;
+  const T irN = rcp(rN), irK = rcp(rK);
   const T vN[3] = {wN[1] * irN, wN[2] * irN, wN[3] * irN};
   const T vK[3] = {wK[1] * irK, wK[2] * irK, wK[3] * irK};
   const T pN = b.gm1 * (wN[4] - T(0.5) * (wN[1] * vN[0] + wN[2] * vN[1] + wN[3] * vN[2]));   // p - p0
@@ -213,8 +221,8 @@ __device__ inline void face_flux(const T wK[5], const T wN[5], const T *q, T sgn
   const T hN = b.H0 + wN[4] + pN, hK = b.H0 + wK[4] + pK;                                   // E + p
   const T HN = hN * irN, HK = hK * irK;
   const T qN = sgn * vN[D], qK = sgn * vK[D];
-  const T s = sqrt(rN * irK);
-  const T iden = T(1) / (s + T(1));
+  const T s = sqrt_(rN * irK);
+  const T iden = rcp(s + T(1));
   T vt[3];
 #pragma unroll
   for (int i = 0; i < 3; ++i) vt[i] = (s * vN[i] + vK[i]) * iden;
@@ -222,7 +230,7 @@ __device__ inline void face_flux(const T wK[5], const T wN[5], const T *q, T sgn
   const T q2 = vt[0] * vt[0] + vt[1] * vt[1] + vt[2] * vt[2];
   const T qt = sgn * vt[D];
   const T c2 = b.gm1 * (ht - T(0.5) * q2);
-  const T c = sqrt(c2), ic = T(1) / c;
+  const T c = sqrt_(c2), ic = rcp(c);
   const T l0 = fabs(qt), lp = fabs(qt + c), lm = fabs(qt - c);
   T dd[5];
 #pragma unroll
@@ -395,9 +403,9 @@ __global__ void __launch_bounds__(NT, (GradWaves<N, E, TQ>::value)) ldg_grad_ker
 // Kernel B: face fluxes + volume integrals + inverse mass + RK epilogue.
 //
 // LDS per element: a 20-field volume buffer of type T (which first serves
-// as the face buffer of one direction: 2 sides x 25 fields x N^2 points),
-// the face residual (5 x N^3, type T), and for T = float an FP64 buffer for
-// the inverse-mass sweeps (for T = double those reuse the volume buffer).
+// as the face buffer of one direction: 2 sides x 25 fields x N^2 points,
+// and last holds the FP64 inverse-mass sweeps: 5 FP64 fields fit in the
+// space of 10 FP32 fields) and the face residual (5 x N^3, type T).
 // ---------------------------------------------------------------------
 template <int N, typename T>
 struct RhsLds {
@@ -407,9 +415,8 @@ struct RhsLds {
   static constexpr int SV = NV * NN;
   static constexpr int FB = 2 * NFF * NF;
   static constexpr int BUF = SV > FB ? SV : FB;      // per element, even for every N
-  static constexpr bool SEP = sizeof(T) != sizeof(double);   // separate FP64 buffer
   // occupancy target (waves per SIMD) handed to the register allocator
-  static constexpr int WAVES = SEP ? (N <= 5 ? 4 : 2) : (N <= 2 ? 4 : N <= 5 ? 3 : 2);
+  static constexpr int WAVES = sizeof(T) == 4 ? (N <= 5 ? 4 : 2) : (N <= 2 ? 4 : N <= 5 ? 3 : 2);
 };
 
 // Operators in LDS: the compute-type copies used by the sweeps and the FP64
@@ -601,13 +608,6 @@ __device__ inline void rhs_face_dir(const double *__restrict__ U, const T *__res
   __syncthreads();
 }
 
-template <int N, int E, typename T, bool SEP>
-struct Fp64Buf;
-template <int N, int E, typename T>
-struct Fp64Buf<N, E, T, true> { double v[E][NSTATE * N * N * N]; };
-template <int N, int E, typename T>
-struct Fp64Buf<N, E, T, false> { double v[1][1]; };
-
 template <int N, int E, typename T>
 __global__ void __launch_bounds__(NT, (RhsLds<N, T>::WAVES))
 ldg_rhs_kernel(const double *__restrict__ U, const T *__restrict__ Q, LaunchArgs la,
@@ -616,7 +616,6 @@ ldg_rhs_kernel(const double *__restrict__ U, const T *__restrict__ Q, LaunchArgs
   constexpr int NN = L::NN, NF = L::NF, NV = L::NV, BUF = L::BUF;
   __shared__ __align__(16) T sbuf[E][BUF];
   __shared__ T sr[E][NSTATE][NN];
-  __shared__ Fp64Buf<N, E, T, L::SEP> smx;
   __shared__ double sred[NT];
   __shared__ SOps<N, T> sops;
   __shared__ long snb[E][6];
@@ -632,11 +631,9 @@ ldg_rhs_kernel(const double *__restrict__ U, const T *__restrict__ Q, LaunchArgs
   const int tid = threadIdx.x;
   const SOps<N, T> *so = &sops;
   const Bg<T> bg(ph);
-  // FP64 view used by the inverse-mass sweeps and the epilogue
-  auto md = [&](int e) -> double * {
-    if constexpr (L::SEP) return smx.v[e];
-    else return reinterpret_cast<double *>(&sbuf[e][0]);
-  };
+  // FP64 view of an element's volume buffer, used by the inverse-mass
+  // sweeps and the epilogue
+  auto md = [&](int e) -> double * { return reinterpret_cast<double *>(&sbuf[e][0]); };
 
   fill_neighbors<E>(m, kl, d.elem_off, ne, snb, tid);
   load_sops<N, T>(&sops, opp, tid);
@@ -835,37 +832,56 @@ ldg_rhs_kernel(const double *__restrict__ U, const T *__restrict__ Q, LaunchArgs
   }
   __syncthreads();
   // x sweep: r = D^T F_x + I^T G + face residual (T), then M1^{-1} along x
-  // in FP64.  With T = double the result overwrites the state fields 0..4
-  // of the same element, which no thread reads any more at this point.
+  // in FP64.  The FP64 result goes to the start of the element's buffer
+  // (5 FP64 fields), which overlaps the F_x fields still being read by other
+  // threads when T = float: every thread keeps its results in registers
+  // until all reads are done.
   {
+    constexpr int XI = (E * NSTATE * NF + NT - 1) / NT;
+    const int tot = ne * NSTATE * NF;
+    double res[XI][N];
     Mat<N, T> I;
     I.load(so->I);
-    for (int w = tid; w < ne * NSTATE * NF; w += NT) {
-      const int e = w / (NSTATE * NF);
-      const int r = w - e * (NSTATE * NF);
-      const int s = r / NF;
-      const int ab = r % NF;
-      const int base = N * ab;
-      const T *px = &sbuf[e][(NSTATE + s) * NN + base];
-      const T *pg = &sbuf[e][(NSTATE + 5 + s) * NN + base];
-      T vx[N], vg[N];
-      double rr[N];
 #pragma unroll
-      for (int q = 0; q < N; ++q) { vx[q] = px[q]; vg[q] = pg[q]; }
+    for (int it = 0; it < XI; ++it) {
+      const int w = tid + it * NT;
+      if (w < tot) {
+        const int e = w / (NSTATE * NF);
+        const int r = w - e * (NSTATE * NF);
+        const int s = r / NF;
+        const int base = N * (r % NF);
+        const T *px = &sbuf[e][(NSTATE + s) * NN + base];
+        const T *pg = &sbuf[e][(NSTATE + 5 + s) * NN + base];
+        T vx[N], vg[N];
+        double rr[N];
 #pragma unroll
-      for (int i = 0; i < N; ++i) {
-        T acc = sr[e][s][base + i];
+        for (int q = 0; q < N; ++q) { vx[q] = px[q]; vg[q] = pg[q]; }
 #pragma unroll
-        for (int q = 0; q < N; ++q) acc += so->D[q][i] * vx[q] + I.v[q][i] * vg[q];
-        rr[i] = (double)acc;
+        for (int i = 0; i < N; ++i) {
+          T acc = sr[e][s][base + i];
+#pragma unroll
+          for (int q = 0; q < N; ++q) acc += so->D[q][i] * vx[q] + I.v[q][i] * vg[q];
+          rr[i] = (double)acc;
+        }
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+          double acc = 0.0;
+#pragma unroll
+          for (int j = 0; j < N; ++j) acc += so->Minv[i][j] * rr[j];
+          res[it][i] = acc;
+        }
       }
-      double *po = md(e) + s * NN + base;
+    }
+    __syncthreads();
 #pragma unroll
-      for (int i = 0; i < N; ++i) {
-        double acc = 0.0;
+    for (int it = 0; it < XI; ++it) {
+      const int w = tid + it * NT;
+      if (w < tot) {
+        const int e = w / (NSTATE * NF);
+        const int r = w - e * (NSTATE * NF);
+        double *po = md(e) + (r / NF) * NN + N * (r % NF);
 #pragma unroll
-        for (int j = 0; j < N; ++j) acc += so->Minv[i][j] * rr[j];
-        po[i] = acc;
+        for (int i = 0; i < N; ++i) po[i] = res[it][i];
       }
     }
   }
