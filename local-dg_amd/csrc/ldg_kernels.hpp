@@ -886,6 +886,37 @@ ldg_rhs_kernel(const double *__restrict__ U, const T *__restrict__ Q, LaunchArgs
     }
   }
   __syncthreads();
+  // FP64 layout: the RK epilogue's HBM operands (u^n and the running sum)
+  // are copied straight into the part of the volume buffer that is free from
+  // here on (after the 5 FP64 result fields) by LDS-direct loads, so their
+  // latency is covered by the inverse-mass sweeps below (1.5 % faster at
+  // 64^3, p = 4).  The FP32 layout has room for one of the two only, and
+  // there the copies cost more than they save: its epilogue loads directly.
+  const bool need_u0 = st.acc || st.unext;
+  const bool need_acc = st.acc && !st.acc_init;
+  constexpr bool PF = sizeof(T) == sizeof(double);
+  auto pf_u0 = [&](int e) -> double * { return md(e) + NSTATE * NN; };
+  auto pf_acc = [&](int e) -> double * { return md(e) + 2 * NSTATE * NN; };
+  if (PF) {
+    constexpr int DW = 2 * NSTATE * NN;    // dwords per element and array
+    const int lane = tid & 63, wave = tid >> 6;
+    for (int e = 0; e < ne; ++e) {
+      const long gofs_e = (k0 + e) * NSTATE * NN;
+      for (int c = wave; c * 64 < DW; c += NT / 64) {
+        const int i = c * 64 + lane;
+        if (i < DW) {
+          if (need_u0)
+            __builtin_amdgcn_global_load_lds(reinterpret_cast<const unsigned *>(st.u0 + gofs_e) + i,
+                                             (__attribute__((address_space(3))) void *)(reinterpret_cast<unsigned *>(pf_u0(e)) + c * 64),
+                                             4, 0, 0);
+          if (need_acc)
+            __builtin_amdgcn_global_load_lds(reinterpret_cast<const unsigned *>(st.acc + gofs_e) + i,
+                                             (__attribute__((address_space(3))) void *)(reinterpret_cast<unsigned *>(pf_acc(e)) + c * 64),
+                                             4, 0, 0);
+        }
+      }
+    }
+  }
   // M1^{-1} along y, then along z (FP64)
 #pragma unroll
   for (int dd = 1; dd < 3; ++dd) {
@@ -914,14 +945,19 @@ ldg_rhs_kernel(const double *__restrict__ U, const T *__restrict__ Q, LaunchArgs
     __syncthreads();
   }
 
+  // the LDS-direct loads above have landed once this wave's memory counter
+  // is zero; the barrier makes every wave's copies visible
+  if (PF) {
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+  }
+
   // ---- epilogue (FP64): k, RK4 accumulation and next stage input.  All of
   // a thread's HBM reads are issued before any is used.
   {
     constexpr int EPI = (E * NSTATE * NN + NT - 1) / NT;
     const int tot = ne * NSTATE * NN;
     const long gofs = k0 * NSTATE * NN;
-    const bool need_u0 = st.acc || st.unext;
-    const bool need_acc = st.acc && !st.acc_init;
     double ru[EPI], ra[EPI], kv[EPI];
 #pragma unroll
     for (int it = 0; it < EPI; ++it) {
@@ -930,9 +966,9 @@ ldg_rhs_kernel(const double *__restrict__ U, const T *__restrict__ Q, LaunchArgs
       ra[it] = 0.0;
       kv[it] = 0.0;
       if (w < tot) {
-        if (need_u0) ru[it] = st.u0[gofs + w];
-        if (need_acc) ra[it] = st.acc[gofs + w];
         const int e = w / (NSTATE * NN), r = w - e * (NSTATE * NN);
+        if (need_u0) ru[it] = PF ? pf_u0(e)[r] : st.u0[gofs + w];
+        if (need_acc) ra[it] = PF ? pf_acc(e)[r] : st.acc[gofs + w];
         kv[it] = md(e)[r];
         if (st.corr && st.gate[k0 + e]) kv[it] += st.corr[gofs + w];
       }
