@@ -210,13 +210,22 @@ int ldg_closure_add_samples(ldg_closure *c, const float *X, const float *T, long
 int ldg_closure_samples(const ldg_closure *c, long *n);
 int ldg_closure_get_samples(ldg_closure *c, long first, long n, float *X, float *T);
 
+/* Loss weighting, chosen before the normalisation is fitted: mode 0
+   (default) weighs every sample alike; mode 1 weighs each collection (one
+   ldg_closure_collect or ldg_closure_add_samples call: one snapshot) by the
+   inverse of its mean normalised target energy, so that snapshots with small
+   corrections count as much as those with large ones.  Weights average 1;
+   ldg_closure_get_weights reads them after the fit. */
+int ldg_closure_set_weighting(ldg_closure *c, int mode);
+int ldg_closure_get_weights(ldg_closure *c, long first, long n, float *w);
+
 /* Per-feature mean/std of the collected samples become the normalisation;
-   the dataset is normalised in place (once). */
+   the dataset is normalised in place (once) and the loss weights are set. */
 int ldg_closure_fit_normalization(ldg_closure *c);
 
-/* Adam on minibatches of the first (1 - val_fraction) of the samples;
-   losses[2 e] / losses[2 e + 1] = training / validation MSE of epoch e
-   (normalised units). */
+/* Adam on minibatches of a seeded random (1 - val_fraction) of the samples;
+   losses[2 e] / losses[2 e + 1] = training / hold-out weighted MSE of epoch
+   e (normalised units). */
 int ldg_closure_train(ldg_closure *c, int epochs, int batch, double lr, double val_fraction, unsigned long long seed,
                       double *losses);
 

@@ -37,10 +37,14 @@ struct ldg_closure {
   long adam_t = 0;
   long cap = 0, count = 0;
   float *Xd = nullptr, *Td = nullptr;
+  float *Wd = nullptr;              // per-sample loss weight (1 unless weighting by collection)
+  std::vector<int> group_of;        // per sample: the collection (snapshot) it came from
+  int ncollect = 0;
+  int weighting = 0;                // 0: uniform; 1: every collection has the same total weight per unit target energy
   bool normalized = false;
   int wsB = 0;                      // training workspace rows
   float *Xb = nullptr, *Tb = nullptr, *Z1 = nullptr, *H1 = nullptr, *Z2 = nullptr, *H2 = nullptr, *Y = nullptr,
-        *dY = nullptr, *dH = nullptr, *dZ = nullptr;
+        *dY = nullptr, *dH = nullptr, *dZ = nullptr, *Wbat = nullptr;
   long *idx = nullptr;
   double *part = nullptr;
   hipStream_t stream = nullptr;
@@ -106,13 +110,13 @@ void forward(const ldg_closure *c, hipStream_t s, int B, const float *X, float *
 
 // Gradient of the mean squared error over rows [0, B) into c->grads; loss
 // partials into c->part.  Uses the training workspace.
-void backward(ldg_closure *c, hipStream_t s, int B, const float *X, const float *T) {
+void backward(ldg_closure *c, hipStream_t s, int B, const float *X, const float *T, const float *rw) {
   const Offs o = offsets(c);
   const float *P = c->params;
   float *G = c->grads;
   forward(c, s, B, X, c->Z1, c->H1, c->Z2, c->H2, c->Y);
   const long ny = (long)B * c->nout;
-  hipLaunchKernelGGL(mlp_loss_grad, dim3(kLossBlocks), dim3(256), 0, s, (const float *)c->Y, T, ny,
+  hipLaunchKernelGGL(mlp_loss_grad, dim3(kLossBlocks), dim3(256), 0, s, (const float *)c->Y, T, ny, c->nout, rw,
                      2.0f / (float)ny, c->dY, c->part);
   gemm<true, false, EPI_NONE>(s, c->nout, c->hidden, B, c->dY, c->nout, c->H2, c->hidden, nullptr, nullptr, G + o.W3,
                               c->hidden);
@@ -139,7 +143,7 @@ void backward(ldg_closure *c, hipStream_t s, int B, const float *X, const float 
 
 int ensure_workspace(ldg_closure *c, int B) {
   if (B <= c->wsB) return LDG_OK;
-  float **bufs[] = {&c->Xb, &c->Tb, &c->Z1, &c->H1, &c->Z2, &c->H2, &c->Y, &c->dY, &c->dH, &c->dZ};
+  float **bufs[] = {&c->Xb, &c->Tb, &c->Z1, &c->H1, &c->Z2, &c->H2, &c->Y, &c->dY, &c->dH, &c->dZ, &c->Wbat};
   for (float **b : bufs) {
     if (*b) (void)hipFree(*b);
     *b = nullptr;
@@ -149,8 +153,8 @@ int ensure_workspace(ldg_closure *c, int B) {
   c->wsB = 0;
   const size_t r = (size_t)B;
   const size_t widths[] = {(size_t)c->nin, (size_t)c->nout, (size_t)c->hidden, (size_t)c->hidden, (size_t)c->hidden,
-                           (size_t)c->hidden, (size_t)c->nout, (size_t)c->nout, (size_t)c->hidden, (size_t)c->hidden};
-  for (int i = 0; i < 10; ++i)
+                           (size_t)c->hidden, (size_t)c->nout, (size_t)c->nout, (size_t)c->hidden, (size_t)c->hidden, 1};
+  for (int i = 0; i < 11; ++i)
     if (hipMalloc((void **)bufs[i], r * widths[i] * sizeof(float)) != hipSuccess)
       return cfail(c, LDG_ENOMEM, "device allocation failed (closure workspace)");
   if (hipMalloc((void **)&c->idx, r * sizeof(long)) != hipSuccess)
@@ -256,8 +260,8 @@ void launch_infer(hipStream_t s, const ldg_closure *c, const double *U, const TQ
 }
 
 void free_closure(ldg_closure *c) {
-  float *bufs[] = {c->params, c->grads, c->m, c->v, c->norm, c->Xd, c->Td, c->Xb, c->Tb, c->Z1,
-                   c->H1, c->Z2, c->H2, c->Y, c->dY, c->dH, c->dZ};
+  float *bufs[] = {c->params, c->grads, c->m, c->v, c->norm, c->Xd, c->Td, c->Wd, c->Xb, c->Tb, c->Z1,
+                   c->H1, c->Z2, c->H2, c->Y, c->dY, c->dH, c->dZ, c->Wbat};
   for (float *b : bufs)
     if (b) (void)hipFree(b);
   if (c->idx) (void)hipFree(c->idx);
@@ -351,7 +355,8 @@ int ldg_closure_create(int p, int hidden, long capacity, unsigned long long seed
                                                            (size_t)c->noutp * c->hidden)) != hipSuccess)
     return bad(LDG_ENOMEM, "device allocation failed (closure parameters)");
   if (capacity > 0 && (hipMalloc((void **)&c->Xd, (size_t)capacity * c->nin * sizeof(float)) != hipSuccess ||
-                       hipMalloc((void **)&c->Td, (size_t)capacity * c->nout * sizeof(float)) != hipSuccess))
+                       hipMalloc((void **)&c->Td, (size_t)capacity * c->nout * sizeof(float)) != hipSuccess ||
+                       hipMalloc((void **)&c->Wd, (size_t)capacity * sizeof(float)) != hipSuccess))
     return bad(LDG_ENOMEM, "device allocation failed (closure dataset)");
   // PyTorch-style uniform initialisation, bound 1/sqrt(fan_in); the output
   // layer scaled by 0.1 so that training starts near a zero correction
@@ -439,7 +444,7 @@ int ldg_closure_gradient(ldg_closure *c, const float *X, const float *T, int B, 
   if (rc) return rc;
   CCHECK(c, hipMemcpyAsync(c->Xb, X, (size_t)B * c->nin * sizeof(float), hipMemcpyHostToDevice, c->stream));
   CCHECK(c, hipMemcpyAsync(c->Tb, T, (size_t)B * c->nout * sizeof(float), hipMemcpyHostToDevice, c->stream));
-  backward(c, c->stream, B, c->Xb, c->Tb);
+  backward(c, c->stream, B, c->Xb, c->Tb, nullptr);
   CCHECK(c, hipGetLastError());
   if (grad) CCHECK(c, hipMemcpyAsync(grad, c->grads, c->nparam * sizeof(float), hipMemcpyDeviceToHost, c->stream));
   std::vector<double> part(kLossBlocks);
@@ -544,8 +549,12 @@ int ldg_closure_collect(ldg_closure *c, ldg_handle *fine, ldg_handle *coarse, do
     (void)hipFree(didx);
     if (dident) (void)hipFree(dident);
     if (e != hipSuccess) return cfail(c, LDG_EDEVICE, std::string("collection: ") + hipGetErrorString(e));
-    if (take > 0) c->count += take;
+    if (take > 0) {
+      c->group_of.insert(c->group_of.end(), take, c->ncollect);
+      c->count += take;
+    }
   }
+  ++c->ncollect;
   return LDG_OK;
 }
 
@@ -555,7 +564,22 @@ int ldg_closure_add_samples(ldg_closure *c, const float *X, const float *T, long
   if (c->count + n > c->cap) return cfail(c, LDG_EINVAL, "dataset capacity exceeded");
   CCHECK(c, hipMemcpy(c->Xd + (size_t)c->count * c->nin, X, (size_t)n * c->nin * sizeof(float), hipMemcpyHostToDevice));
   CCHECK(c, hipMemcpy(c->Td + (size_t)c->count * c->nout, T, (size_t)n * c->nout * sizeof(float), hipMemcpyHostToDevice));
+  c->group_of.insert(c->group_of.end(), n, c->ncollect++);
   c->count += n;
+  return LDG_OK;
+}
+
+int ldg_closure_set_weighting(ldg_closure *c, int mode) {
+  if (!c || mode < 0 || mode > 1) return LDG_EINVAL;
+  if (c->normalized) return cfail(c, LDG_EINVAL, "choose the weighting before fitting the normalisation");
+  c->weighting = mode;
+  return LDG_OK;
+}
+
+int ldg_closure_get_weights(ldg_closure *c, long first, long n, float *w) {
+  if (!c || !w || first < 0 || n < 0 || first + n > c->count) return LDG_EINVAL;
+  if (!c->normalized) return cfail(c, LDG_EINVAL, "weights are set when the normalisation is fitted");
+  CCHECK(c, hipMemcpy(w, c->Wd + first, (size_t)n * sizeof(float), hipMemcpyDeviceToHost));
   return LDG_OK;
 }
 
@@ -620,6 +644,30 @@ int ldg_closure_fit_normalization(ldg_closure *c) {
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   (void)hipFree(doinv);
   if (e != hipSuccess) return cfail(c, LDG_EDEVICE, "normalisation failed");
+  // loss weights: uniform, or 1 / (mean normalised target energy of the
+  // sample's collection), scaled to mean 1 over the samples
+  std::vector<float> w(c->count, 1.0f);
+  if (c->weighting == 1) {
+    float *ms = nullptr;
+    CCHECK(c, hipMalloc((void **)&ms, c->count * sizeof(float)));
+    hipLaunchKernelGGL(mlp_rowmeansq, dim3(grid_for(c->count)), dim3(256), 0, c->stream, (const float *)c->Td,
+                       c->count, c->nout, ms);
+    std::vector<float> hm(c->count);
+    e = hipMemcpyAsync(hm.data(), ms, c->count * sizeof(float), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(ms);
+    if (e != hipSuccess) return cfail(c, LDG_EDEVICE, "weights failed");
+    std::vector<double> gs(c->ncollect, 0.0), gn(c->ncollect, 0.0);
+    for (long i = 0; i < c->count; ++i) { gs[c->group_of[i]] += hm[i]; gn[c->group_of[i]] += 1.0; }
+    double tot = 0.0;
+    for (long i = 0; i < c->count; ++i) {
+      const double m = gs[c->group_of[i]] / gn[c->group_of[i]];
+      w[i] = (float)(m > 0 ? 1.0 / m : 1.0);
+      tot += w[i];
+    }
+    for (long i = 0; i < c->count; ++i) w[i] = (float)(w[i] * (c->count / tot));
+  }
+  CCHECK(c, hipMemcpy(c->Wd, w.data(), c->count * sizeof(float), hipMemcpyHostToDevice));
   c->normalized = true;
   return LDG_OK;
 }
@@ -632,25 +680,38 @@ int ldg_closure_train(ldg_closure *c, int epochs, int batch, double lr, double v
   if (ntrain < batch) return cfail(c, LDG_EINVAL, "fewer training samples than one batch");
   int rc = ensure_workspace(c, batch);
   if (rc) return rc;
+  // a seeded random hold-out of nval samples; the rest is reshuffled every epoch
   std::mt19937_64 rng(seed);
-  std::vector<long> perm(ntrain);
-  std::iota(perm.begin(), perm.end(), 0L);
-  long *didx = nullptr;
+  std::vector<long> all(c->count);
+  std::iota(all.begin(), all.end(), 0L);
+  std::shuffle(all.begin(), all.end(), rng);
+  std::vector<long> perm(all.begin(), all.begin() + ntrain);
+  long *didx = nullptr, *dval = nullptr;
   CCHECK(c, hipMalloc((void **)&didx, std::max<long>(ntrain, 1) * sizeof(long)));
+  if (hipMalloc((void **)&dval, std::max<long>(nval, 1) * sizeof(long)) != hipSuccess) {
+    (void)hipFree(didx);
+    return cfail(c, LDG_ENOMEM, "device allocation failed (training indices)");
+  }
+  hipError_t e = nval ? hipMemcpy(dval, all.data() + ntrain, nval * sizeof(long), hipMemcpyHostToDevice) : hipSuccess;
+  const float *wsrc = c->normalized ? c->Wd : nullptr;     // per-sample weights (set with the normalisation)
   std::vector<double> part(kLossBlocks);
   const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
-  hipError_t e = hipSuccess;
+  auto gather = [&](const long *idx, int rows) {
+    hipLaunchKernelGGL(mlp_gather, dim3(grid_for((long)rows * c->nin)), dim3(256), 0, c->stream,
+                       (const float *)c->Xd, c->nin, idx, rows, c->Xb);
+    hipLaunchKernelGGL(mlp_gather, dim3(grid_for((long)rows * c->nout)), dim3(256), 0, c->stream,
+                       (const float *)c->Td, c->nout, idx, rows, c->Tb);
+    if (wsrc)
+      hipLaunchKernelGGL(mlp_gather, dim3(grid_for(rows)), dim3(256), 0, c->stream, wsrc, 1, idx, rows, c->Wbat);
+  };
   for (int ep = 0; ep < epochs && e == hipSuccess; ++ep) {
     std::shuffle(perm.begin(), perm.end(), rng);
     e = hipMemcpy(didx, perm.data(), ntrain * sizeof(long), hipMemcpyHostToDevice);
     double tl = 0.0;
     long nb = 0;
     for (long b0 = 0; b0 + batch <= ntrain && e == hipSuccess; b0 += batch, ++nb) {
-      hipLaunchKernelGGL(mlp_gather, dim3(grid_for((long)batch * c->nin)), dim3(256), 0, c->stream,
-                         (const float *)c->Xd, c->nin, (const long *)(didx + b0), batch, c->Xb);
-      hipLaunchKernelGGL(mlp_gather, dim3(grid_for((long)batch * c->nout)), dim3(256), 0, c->stream,
-                         (const float *)c->Td, c->nout, (const long *)(didx + b0), batch, c->Tb);
-      backward(c, c->stream, batch, c->Xb, c->Tb);
+      gather(didx + b0, batch);
+      backward(c, c->stream, batch, c->Xb, c->Tb, wsrc ? c->Wbat : nullptr);
       ++c->adam_t;
       const float bc1 = 1.0f - (float)std::pow(b1, (double)c->adam_t), bc2 = 1.0f - (float)std::pow(b2, (double)c->adam_t);
       hipLaunchKernelGGL(mlp_adam, dim3(grid_for(c->nparam)), dim3(256), 0, c->stream, c->params,
@@ -661,11 +722,13 @@ int ldg_closure_train(ldg_closure *c, int epochs, int batch, double lr, double v
     }
     double vl = 0.0;
     long vn = 0;
-    for (long v0 = ntrain; v0 < c->count && e == hipSuccess; v0 += batch) {
-      const int rows = (int)std::min<long>(batch, c->count - v0);
-      forward(c, c->stream, rows, c->Xd + v0 * c->nin, nullptr, c->H1, nullptr, c->H2, c->Y);
+    for (long v0 = 0; v0 < nval && e == hipSuccess; v0 += batch) {
+      const int rows = (int)std::min<long>(batch, nval - v0);
+      gather(dval + v0, rows);
+      forward(c, c->stream, rows, c->Xb, nullptr, c->H1, nullptr, c->H2, c->Y);
       hipLaunchKernelGGL(mlp_loss_grad, dim3(kLossBlocks), dim3(256), 0, c->stream, (const float *)c->Y,
-                         (const float *)(c->Td + v0 * c->nout), (long)rows * c->nout, 0.0f, (float *)nullptr, c->part);
+                         (const float *)c->Tb, (long)rows * c->nout, c->nout,
+                         (const float *)(wsrc ? c->Wbat : nullptr), 0.0f, (float *)nullptr, c->part);
       e = hipMemcpyAsync(part.data(), c->part, kLossBlocks * sizeof(double), hipMemcpyDeviceToHost, c->stream);
       if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
       vl += std::accumulate(part.begin(), part.end(), 0.0);
@@ -677,6 +740,7 @@ int ldg_closure_train(ldg_closure *c, int epochs, int batch, double lr, double v
     }
   }
   (void)hipFree(didx);
+  (void)hipFree(dval);
   if (e != hipSuccess) return cfail(c, LDG_EDEVICE, std::string("training: ") + hipGetErrorString(e));
   return pack_bf16(c);
 }

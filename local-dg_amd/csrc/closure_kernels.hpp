@@ -185,16 +185,18 @@ __global__ void __launch_bounds__(64) mlp_correction(const float *__restrict__ Y
   }
 }
 
-// dY = scale (Y - T); per-block partial sums of (Y - T)^2 (FP64).
+// dY = scale rw (Y - T); per-block partial sums of rw (Y - T)^2 (FP64),
+// with rw the weight of the row (rows of `width` values; 1 if rw is null).
 __global__ void __launch_bounds__(256) mlp_loss_grad(const float *__restrict__ Y, const float *__restrict__ T,
-                                                     long n, float scale, float *__restrict__ dY,
-                                                     double *__restrict__ part) {
+                                                     long n, int width, const float *__restrict__ rw, float scale,
+                                                     float *__restrict__ dY, double *__restrict__ part) {
   __shared__ double red[256];
   double s = 0.0;
   for (long w = blockIdx.x * 256L + threadIdx.x; w < n; w += (long)gridDim.x * 256) {
     const float d = Y[w] - T[w];
-    if (dY) dY[w] = scale * d;
-    s += (double)d * d;
+    const float r = rw ? rw[w / width] : 1.0f;
+    if (dY) dY[w] = scale * r * d;
+    s += (double)r * d * d;
   }
   red[threadIdx.x] = s;
   __syncthreads();
@@ -246,6 +248,15 @@ __global__ void mlp_gather(const float *__restrict__ src, int width, const long 
     const long b = w / width;
     const int j = (int)(w - b * width);
     dst[w] = src[idx[b] * width + j];
+  }
+}
+
+// out[r] = mean over the row of D[r][j]^2 (one thread per row).
+__global__ void mlp_rowmeansq(const float *__restrict__ D, long rows, int width, float *__restrict__ out) {
+  for (long r = blockIdx.x * (long)blockDim.x + threadIdx.x; r < rows; r += (long)gridDim.x * blockDim.x) {
+    double s = 0.0;
+    for (int j = 0; j < width; ++j) s += (double)D[r * width + j] * D[r * width + j];
+    out[r] = (float)(s / width);
   }
 }
 

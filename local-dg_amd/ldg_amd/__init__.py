@@ -38,7 +38,7 @@ EXPORTED = [
     "ldg_closure_get", "ldg_closure_set", "ldg_closure_forward", "ldg_closure_gradient",
     "ldg_project_state", "ldg_closure_collect", "ldg_closure_add_samples", "ldg_closure_samples",
     "ldg_closure_get_samples", "ldg_closure_fit_normalization", "ldg_closure_train", "ldg_set_closure",
-    "ldg_closure_gated",
+    "ldg_closure_gated", "ldg_closure_set_weighting", "ldg_closure_get_weights",
 ]
 
 
@@ -131,6 +131,8 @@ def load_library(path=LIB_PATH):
                                     ctypes.c_ulonglong, _dp]
     L.ldg_set_closure.argtypes = [h, h, ctypes.c_double, ctypes.c_int]
     L.ldg_closure_gated.argtypes = [h, lp, lp]
+    L.ldg_closure_set_weighting.argtypes = [h, ctypes.c_int]
+    L.ldg_closure_get_weights.argtypes = [h, ctypes.c_long, ctypes.c_long, fp]
     for name in EXPORTED:
         fn = getattr(L, name)
         if name not in ("ldg_destroy", "ldg_last_error", "ldg_time_step", "ldg_closure_destroy",

@@ -138,6 +138,17 @@ class Closure:
         self._check(self._lib.ldg_closure_get_samples(self._c, first, n, _f(X), _f(T)))
         return X, T
 
+    def set_weighting(self, mode):
+        """0: uniform; 1: every collection (snapshot) weighted by the inverse
+        of its mean normalised target energy."""
+        self._check(self._lib.ldg_closure_set_weighting(self._c, int(mode)))
+
+    def get_weights(self, first=0, n=None):
+        n = self.samples - first if n is None else n
+        w = np.empty(n, dtype=np.float32)
+        self._check(self._lib.ldg_closure_get_weights(self._c, first, n, _f(w)))
+        return w
+
     def fit_normalization(self):
         self._check(self._lib.ldg_closure_fit_normalization(self._c))
 
@@ -334,6 +345,7 @@ def main(argv=None):
     tr.add_argument("--batch", type=int, default=2048)
     tr.add_argument("--lr", type=float, default=1e-3)
     tr.add_argument("--seed", type=int, default=0)
+    tr.add_argument("--weighting", choices=["uniform", "snapshot"], default="uniform")
     tr.add_argument("--out", default="closure.npz")
     ev = sub.add_parser("evaluate")
     ev.add_argument("--weights", required=True)
@@ -364,6 +376,8 @@ def main(argv=None):
         cap = int(a.runs * nsnap * math.ceil(a.keep * ne ** 3)) + 16
         c = Closure(2, 256, cap, a.seed)
         info = generate(c, a.runs, a.p_fine, ne, t_end, a.dt_snap, a.keep, a.cfl, seed=a.seed, log=log)
+        c.set_weighting(1 if a.weighting == "snapshot" else 0)
+        info["weighting"] = a.weighting
         c.fit_normalization()
         t0 = time.perf_counter()
         losses = c.train(a.epochs, a.batch, a.lr, 0.1, a.seed)

@@ -233,3 +233,31 @@ def test_dissipative_gate_keeps_only_energy_removing_corrections(hidden):
     d = k1 - k0
     assert np.max(np.abs(d[~on])) == 0.0
     assert rel(d[on], full[on]) <= 1e-4
+
+
+def test_snapshot_weighting():
+    """Mode 1 weighs each collection by the inverse of its mean normalised
+    target energy (weights averaging 1); mode 0 leaves every weight 1."""
+    rng = np.random.default_rng(6)
+    n = 600
+    X = rng.standard_normal((n, 540)).astype(np.float32)
+    T1 = rng.standard_normal((n, 135)).astype(np.float32)
+    T2 = (0.05 * rng.standard_normal((n, 135)) + 0.3).astype(np.float32)
+    for mode in (0, 1):
+        c = C.Closure(2, 64, 2 * n)
+        c.add_samples(X, T1)
+        c.add_samples(X, T2)
+        c.set_weighting(mode)
+        c.fit_normalization()
+        w = c.get_weights()
+        if mode == 0:
+            assert np.all(w == 1.0)
+            continue
+        _, Tn = c.get_samples()
+        m1 = np.mean(Tn[:n].astype(np.float64) ** 2)
+        m2 = np.mean(Tn[n:].astype(np.float64) ** 2)
+        assert np.allclose(w[:n], w[0]) and np.allclose(w[n:], w[n])
+        assert abs(w[0] / w[n] - m2 / m1) <= 1e-5 * (m2 / m1)
+        assert abs(np.mean(w.astype(np.float64)) - 1.0) <= 1e-5
+        losses = c.train(2, batch=128, lr=1e-3, val_fraction=0.25, seed=1)
+        assert np.all(np.isfinite(losses))
