@@ -201,7 +201,18 @@ int build_tiling(ldg_handle *h, Group &g, Tiling &t, int E) {
     d.nx = in.nx; d.ny = in.ny; d.nz = in.nz;
     d.nelem = in.K;
     d.elem_off = off;
-    d.nblocks = (int)((in.K + E - 1) / E);
+    // blocks of E elements along x-rows, visited in bricks of about 8 x 8 x 4
+    // elements (whole bricks; blocks outside the mesh are padding)
+    d.nbx = (in.nx + E - 1) / E;
+    d.tb[0] = std::min(std::max(1, 8 / E), d.nbx);
+    d.tb[1] = std::min(8, in.ny);
+    d.tb[2] = std::min(4, in.nz);
+    d.nbrick[0] = (d.nbx + d.tb[0] - 1) / d.tb[0];
+    d.nbrick[1] = (in.ny + d.tb[1] - 1) / d.tb[1];
+    d.nbrick[2] = (in.nz + d.tb[2] - 1) / d.tb[2];
+    const long nb = (long)d.nbrick[0] * d.nbrick[1] * d.nbrick[2] * d.tb[0] * d.tb[1] * d.tb[2];
+    if (nb > (1L << 30)) return fail(h, LDG_EINVAL, "mesh too large for the block numbering");
+    d.nblocks = (int)nb;
     d.block_off = boff;
     for (int l = 0; l < 3; ++l) d.ialpha[l] = 2.0 / in.dx[l];
     d.dt = 0.0;

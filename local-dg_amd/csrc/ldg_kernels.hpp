@@ -76,15 +76,40 @@ struct StageArgs {
 // instance.  Instances are stored back to back in the launch's arrays.
 struct InstDesc {
   int nx, ny, nz;
-  int nblocks;           // its logical workgroups
+  int nblocks;           // its logical workgroups (whole bricks, see block_elements)
   long nelem;            // elements of this instance
   long elem_off;         // its first element in the launch's arrays
   int block_off;         // its first logical workgroup
-  int pad;
+  int nbx;               // element blocks per x-row: ceil(nx / E)
+  int tb[3];             // brick of tb[0] x tb[1] x tb[2] blocks (x blocks, y rows, z planes)
+  int nbrick[3];         // bricks per direction
   double ialpha[3];      // 2 / dx_l
   double dt;             // its time step
   PhysArgs ph;
 };
+
+// Elements of an instance's logical workgroup lbl (0-based within the
+// instance): blocks are E consecutive elements of one x-row, and blocks are
+// visited brick by brick (tb blocks per brick, bricks x-fastest), so that the
+// workgroups resident at the same time on an XCD cover a compact region and
+// their face neighbours in y and z are mostly in that XCD's L2.  Returns the
+// first element (local index) and the count, 0 for a padding block of a
+// partial brick.
+__host__ __device__ inline void block_elements(const InstDesc &d, int E, int lbl, long &kl, int &ne) {
+  const int bs = d.tb[0] * d.tb[1] * d.tb[2];
+  const int brick = lbl / bs, off = lbl - brick * bs;
+  const int Bx = brick % d.nbrick[0], By = (brick / d.nbrick[0]) % d.nbrick[1], Bz = brick / (d.nbrick[0] * d.nbrick[1]);
+  const int ox = off % d.tb[0], oy = (off / d.tb[0]) % d.tb[1], oz = off / (d.tb[0] * d.tb[1]);
+  const int bx = Bx * d.tb[0] + ox, ey = By * d.tb[1] + oy, ez = Bz * d.tb[2] + oz;
+  if (bx >= d.nbx || ey >= d.ny || ez >= d.nz) {
+    kl = 0;
+    ne = 0;
+    return;
+  }
+  const int ex = bx * E;
+  kl = ex + (long)d.nx * (ey + (long)d.ny * ez);
+  ne = (d.nx - ex) < E ? (d.nx - ex) : E;
+}
 
 struct LaunchArgs {
   const InstDesc *inst;
@@ -110,8 +135,7 @@ __device__ inline const InstDesc &block_instance(const LaunchArgs &la, int lb, M
   m.nx = d.nx; m.ny = d.ny; m.nz = d.nz;
   m.nelem = d.nelem;
   m.ialpha[0] = d.ialpha[0]; m.ialpha[1] = d.ialpha[1]; m.ialpha[2] = d.ialpha[2];
-  kl = (long)(lb - d.block_off) * E;
-  ne = (int)((d.nelem - kl) < E ? (d.nelem - kl) : E);
+  block_elements(d, E, lb - d.block_off, kl, ne);
   return d;
 }
 
@@ -308,6 +332,7 @@ __global__ void __launch_bounds__(NT, (GradWaves<N, E, TQ>::value)) ldg_grad_ker
   long kl;
   int ne;
   const InstDesc &d = block_instance<E>(la, lb, m, kl, ne);
+  if (ne == 0) return;                // padding block of a partial brick
   const long k0 = d.elem_off + kl;    // first element of the block in the arrays
   const int tid = threadIdx.x;
   fill_neighbors<E>(m, kl, d.elem_off, ne, snb, tid);
@@ -625,6 +650,13 @@ ldg_rhs_kernel(const double *__restrict__ U, const T *__restrict__ Q, LaunchArgs
   long kl;
   int ne;
   const InstDesc &d = block_instance<E>(la, lb, m, kl, ne);
+  if (ne == 0) {                      // padding block of a partial brick: empty partial sums
+    if (threadIdx.x == 0) {
+      if (st.ke_part) st.ke_part[lb] = 0.0;
+      if (st.diag_part) { st.diag_part[2 * lb] = 0.0; st.diag_part[2 * lb + 1] = 0.0; }
+    }
+    return;
+  }
   const long k0 = d.elem_off + kl;    // first element of the block in the arrays
   const PhysArgs ph = d.ph;
   const double dtb = d.dt * st.cb, dta = d.dt * st.ca;

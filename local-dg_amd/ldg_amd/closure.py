@@ -265,9 +265,14 @@ def run_tgv(p, ne, t_end, closure=None, bound=5.0, precision=LDG_FP64, cfl=0.01,
     kes, diag = [], [g.diagnostics()]
     wall = 0.0
     off_max = 0
+    diverged = None
     for k in range(ndiag):
         t0 = time.perf_counter()
-        kes.append(g.step(dt, n_per, record_ke=True))
+        try:
+            kes.append(g.step(dt, n_per, record_ke=True))
+        except LDGError:
+            diverged = (k + 1) * dt_diag          # non-finite kinetic energy within this interval
+            break
         wall += time.perf_counter() - t0
         diag.append(g.diagnostics())
         if closure is not None:
@@ -275,10 +280,11 @@ def run_tgv(p, ne, t_end, closure=None, bound=5.0, precision=LDG_FP64, cfl=0.01,
         if log and k % 10 == 9:
             log(f"p={p} t={(k + 1) * dt_diag:.1f} KE={diag[-1]['ke']:.6f} wall={wall:.1f}s")
     g.close()
-    ke = np.concatenate(kes)
+    ke = np.concatenate(kes) if kes else np.zeros(0)
     return {"p": p, "ne": ne, "dt": dt, "steps": int(len(ke)), "ke_ref": ke,
-            "t_diag": np.arange(ndiag + 1) * dt_diag, "ke_gauss": np.array([d["ke"] for d in diag]),
-            "enstrophy": np.array([d["enstrophy"] for d in diag]), "wall_s": wall, "gated_max": off_max}
+            "t_diag": np.arange(len(diag)) * dt_diag, "ke_gauss": np.array([d["ke"] for d in diag]),
+            "enstrophy": np.array([d["enstrophy"] for d in diag]), "wall_s": wall, "gated_max": off_max,
+            "diverged_at": diverged}
 
 
 def dissipation_curve(t, ke):
@@ -288,7 +294,10 @@ def dissipation_curve(t, ke):
 
 def compare(run, ref):
     """Relative max error of -dEk/dt (Gauss-point KE every dt_diag) of run vs
-    ref on ref's diagnostic times, normalised by max |ref|."""
+    ref on ref's diagnostic times, normalised by max |ref| (inf if the run
+    diverged)."""
+    if run.get("diverged_at") is not None or len(run["t_diag"]) < len(ref["t_diag"]):
+        return float("inf")
     d_run = dissipation_curve(run["t_diag"], run["ke_gauss"])
     d_ref = dissipation_curve(ref["t_diag"], ref["ke_gauss"])
     d_run = np.interp(ref["t_diag"], run["t_diag"], d_run)
@@ -415,6 +424,7 @@ def main(argv=None):
                     res[f"wall_ratio_ref_over_{k}"] = ref["wall_s"] / r["wall_s"]
         for k, r in runs.items():
             res[k] = {"wall_s": r["wall_s"], "steps": r["steps"], "gated_max": r["gated_max"],
+                      "diverged_at": r.get("diverged_at"),
                       "t": r["t_diag"].tolist(), "ke_gauss": r["ke_gauss"].tolist(),
                       "dissipation": dissipation_curve(r["t_diag"], r["ke_gauss"]).tolist(),
                       "enstrophy": r["enstrophy"].tolist()}
