@@ -261,3 +261,20 @@ def test_snapshot_weighting():
         assert abs(np.mean(w.astype(np.float64)) - 1.0) <= 1e-5
         losses = c.train(2, batch=128, lr=1e-3, val_fraction=0.25, seed=1)
         assert np.all(np.isfinite(losses))
+
+
+def test_corrected_run_is_deterministic():
+    """Two runs with the same closure attached are bit-identical (no
+    uninitialised memory or order-dependent sums in the corrected stage)."""
+    c = C.Closure(2, 256, 0, seed=21)
+    c.set(norm=random_norm(c, np.random.default_rng(7)))
+    out = []
+    for _ in range(2):
+        g = ldg_amd.LDG(2, 8)
+        C.attach(g, c, 1e30)
+        g.initial_condition()
+        ke = g.step(g.time_step(0.01), 40)
+        out.append((g.get_state(), ke))
+        g.close()
+    np.testing.assert_array_equal(out[0][0], out[1][0])
+    np.testing.assert_array_equal(out[0][1], out[1][1])
