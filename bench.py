@@ -52,15 +52,18 @@ def cpu_baseline(p, ne, steps):
 
 
 def alg_bytes(dof, mixed):
-    """Algorithmic HBM bytes per launch (8-byte state words; the gradient Q
-    is 3 fields per state variable, 4 bytes each when mixed):
+    """Algorithmic HBM bytes per launch, averaged over the 4 RK stages (u^n,
+    the running sum and, in the FP64 path, the stage inputs are 8-byte words;
+    the gradient Q is 3 fields per state variable, 4 bytes each when mixed;
+    in the mixed path the stage inputs of stages 2-4 are 4-byte
+    fluctuations):
       grad: read U_s, write Q
-      rhs : read U_s, Q, u^n; acc read+write; write U_{s+1}; averaged over
-            the 4 stages (stage 1 reads U_s = u^n once and no acc, stage 4
-            writes no next input)."""
+      rhs : read U_s, Q, u^n; acc read+write; write U_{s+1} (stage 1 reads
+            U_s = u^n once and no acc, stage 4 writes no next input)."""
     q = 3 * (4 if mixed else 8)
-    grad = (8 + q) * dof
-    rhs = ((8 + q + 8 + 8) + (8 + q + 8 + 16 + 8) * 2 + (8 + q + 8 + 16)) / 4 * dof
+    u1, us = 8, (4 if mixed else 8)      # stage input: stage 1, stages 2-4 (read and written)
+    grad = ((u1 + q) + 3 * (us + q)) / 4 * dof
+    rhs = ((u1 + q + 8 + us) + (us + q + 8 + 16 + us) * 2 + (us + q + 8 + 16)) / 4 * dof
     return grad, rhs
 
 

@@ -221,42 +221,55 @@ int project_device(hipStream_t s, int pf, int pc, long K, const double *src, dou
   return LDG_OK;
 }
 
+// Background (rho0, E0) of instance in: the state minus it is what the
+// closure sees.
+void background(const ldg_handle *h, const Inst &in, double &rho0, double &E0) {
+  const double c0 = in.V0 / in.M0;
+  const double p0 = h->cfg.rho0 * c0 * c0 / h->cfg.gamma;
+  rho0 = h->cfg.rho0;
+  E0 = p0 / (h->cfg.gamma - 1.0);
+}
+
 // features of one instance's elements into X (rows of the instance)
-template <typename TQ>
-void launch_features(hipStream_t s, const double *U, const TQ *Q, long K, int dofs, double rho0, double E0,
+template <typename TQ, typename UT>
+void launch_features(hipStream_t s, const UT *U, const TQ *Q, long K, int dofs, double rho0, double E0,
                      const float *norm, int nin, float *X) {
-  hipLaunchKernelGGL((mlp_features<TQ>), dim3(grid_for(K * nin)), dim3(256), 0, s, U, Q, K, dofs, rho0, E0, norm,
+  hipLaunchKernelGGL((mlp_features<TQ, UT>), dim3(grid_for(K * nin)), dim3(256), 0, s, U, Q, K, dofs, rho0, E0, norm,
                      norm + nin, X);
 }
 
-void features_group(ldg_handle *h, Group &g, const double *U, const float *norm, int nin, float *X) {
+// U: the group's stage input (state, or FP32 fluctuations: nothing to subtract)
+void features_group(ldg_handle *h, Group &g, StageIn U, const float *norm, int nin, float *X) {
   const int dofs = g.n * g.n * g.n;
   long off = 0;
   for (int idx : g.members) {
     const Inst &in = h->inst[idx];
-    const double c0 = in.V0 / in.M0;
-    const double p0 = h->cfg.rho0 * c0 * c0 / h->cfg.gamma;
-    const double E0 = p0 / (h->cfg.gamma - 1.0);
-    const double *Ui = U + off * 5 * dofs;
-    if (h->mixed)
-      launch_features<float>(h->stream, Ui, (const float *)g.q + off * 15 * dofs, in.K, dofs, h->cfg.rho0, E0, norm,
-                             nin, X + off * nin);
+    double rho0, E0;
+    background(h, in, rho0, E0);
+    if (U.fluct) rho0 = E0 = 0.0;
+    const long uo = off * 5 * dofs;
+    if (h->mixed && U.fluct)
+      launch_features<float, float>(h->stream, (const float *)U.p + uo, (const float *)g.q + off * 15 * dofs, in.K,
+                                    dofs, rho0, E0, norm, nin, X + off * nin);
+    else if (h->mixed)
+      launch_features<float, double>(h->stream, (const double *)U.p + uo, (const float *)g.q + off * 15 * dofs, in.K,
+                                     dofs, rho0, E0, norm, nin, X + off * nin);
     else
-      launch_features<double>(h->stream, Ui, (const double *)g.q + off * 15 * dofs, in.K, dofs, h->cfg.rho0, E0, norm,
-                              nin, X + off * nin);
+      launch_features<double, double>(h->stream, (const double *)U.p + uo, (const double *)g.q + off * 15 * dofs,
+                                      in.K, dofs, rho0, E0, norm, nin, X + off * nin);
     off += in.K;
   }
 }
 
-template <typename TQ>
-void launch_infer(hipStream_t s, const ldg_closure *c, const double *U, const TQ *Q, long K, double rho0, double E0,
-                  double bound, int dissipative, double *corr, unsigned char *gate) {
+template <typename TQ, typename UT>
+void launch_infer(hipStream_t s, const ldg_closure *c, const UT *U, const TQ *Q, long K, double rho0, double E0,
+                  double bound, int dissipative, double rho_add, double *corr, unsigned char *gate) {
   const Offs o = offsets(c);
   const unsigned short *W1b = c->Wb, *W2b = W1b + (long)c->hidden * c->k1p, *W3b = W2b + (long)c->hidden * c->hidden;
-  hipLaunchKernelGGL((mlp_infer<TQ, 540, 256, 135>), dim3((unsigned)((K + FR - 1) / FR)), dim3(256), 0, s, U, Q, K,
-                     rho0, E0, (const float *)c->norm, W1b, W2b, W3b, (const float *)(c->params + o.b1),
+  hipLaunchKernelGGL((mlp_infer<TQ, UT, 540, 256, 135>), dim3((unsigned)((K + FR - 1) / FR)), dim3(256), 0, s, U, Q,
+                     K, rho0, E0, (const float *)c->norm, W1b, W2b, W3b, (const float *)(c->params + o.b1),
                      (const float *)(c->params + o.b2), (const float *)(c->params + o.b3), bound, dissipative,
-                     corr, gate);
+                     rho_add, corr, gate);
 }
 
 void free_closure(ldg_closure *c) {
@@ -275,34 +288,47 @@ void free_closure(ldg_closure *c) {
 
 namespace ldg_host {
 
-int closure_stage(ldg_handle *h, Group &g, const double *U, StageArgs &st) {
+int closure_stage(ldg_handle *h, Group &g, StageIn U, StageArgs &st) {
   GroupClosure &cl = g.cl;
   if (!cl.net) return LDG_OK;
   const ldg_closure *c = cl.net;
+  const int dofs = g.n * g.n * g.n;
   if (fused_ok(c)) {
     // one fused launch per instance (each has its own background state)
-    const int dofs = g.n * g.n * g.n;
     long off = 0;
     for (int idx : g.members) {
       const Inst &in = h->inst[idx];
-      const double c0 = in.V0 / in.M0;
-      const double p0 = h->cfg.rho0 * c0 * c0 / h->cfg.gamma;
-      const double E0 = p0 / (h->cfg.gamma - 1.0);
-      if (h->mixed)
-        launch_infer<float>(h->stream, c, U + off * 5 * dofs, (const float *)g.q + off * 15 * dofs, in.K,
-                            h->cfg.rho0, E0, cl.bound, cl.dissipative, cl.corr + off * c->nout, cl.gate + off);
+      double rho0, E0;
+      background(h, in, rho0, E0);
+      const double rho_add = U.fluct ? rho0 : 0.0;     // density = input + rho_add
+      if (U.fluct) rho0 = E0 = 0.0;
+      const long uo = off * 5 * dofs;
+      if (h->mixed && U.fluct)
+        launch_infer<float, float>(h->stream, c, (const float *)U.p + uo, (const float *)g.q + off * 15 * dofs, in.K,
+                                   rho0, E0, cl.bound, cl.dissipative, rho_add, cl.corr + off * c->nout, cl.gate + off);
+      else if (h->mixed)
+        launch_infer<float, double>(h->stream, c, (const double *)U.p + uo, (const float *)g.q + off * 15 * dofs,
+                                    in.K, rho0, E0, cl.bound, cl.dissipative, rho_add, cl.corr + off * c->nout,
+                                    cl.gate + off);
       else
-        launch_infer<double>(h->stream, c, U + off * 5 * dofs, (const double *)g.q + off * 15 * dofs, in.K,
-                             h->cfg.rho0, E0, cl.bound, cl.dissipative, cl.corr + off * c->nout, cl.gate + off);
+        launch_infer<double, double>(h->stream, c, (const double *)U.p + uo, (const double *)g.q + off * 15 * dofs,
+                                     in.K, rho0, E0, cl.bound, cl.dissipative, rho_add, cl.corr + off * c->nout,
+                                     cl.gate + off);
       off += in.K;
     }
   } else {
     features_group(h, g, U, c->norm, c->nin, cl.X);
     forward(c, h->stream, (int)g.K, cl.X, nullptr, cl.H1, nullptr, cl.H2, cl.Y);
-    hipLaunchKernelGGL(mlp_correction, dim3((unsigned)std::min<long>(g.K, 65535)), dim3(64), 0, h->stream,
-                       (const float *)cl.Y, g.K, c->nout, (const float *)(c->norm + 2 * c->nin),
-                       (const float *)(c->norm + 2 * c->nin + c->nout), cl.bound, cl.dissipative, U, cl.corr,
-                       cl.gate);
+    // the dissipative gate needs the density (rho0 is shared by all instances)
+    const double rho_add = U.fluct ? h->cfg.rho0 : 0.0;
+    const dim3 grid((unsigned)std::min<long>(g.K, 65535)), block(64);
+    const float *om = c->norm + 2 * c->nin, *os = c->norm + 2 * c->nin + c->nout;
+    if (U.fluct)
+      hipLaunchKernelGGL((mlp_correction<float>), grid, block, 0, h->stream, (const float *)cl.Y, g.K, c->nout, om, os,
+                         cl.bound, cl.dissipative, (const float *)U.p, rho_add, cl.corr, cl.gate);
+    else
+      hipLaunchKernelGGL((mlp_correction<double>), grid, block, 0, h->stream, (const float *)cl.Y, g.K, c->nout, om,
+                         os, cl.bound, cl.dissipative, (const double *)U.p, rho_add, cl.corr, cl.gate);
   }
   HIPCHECK(h, hipGetLastError());
   st.corr = cl.corr;
@@ -480,29 +506,29 @@ int ldg_closure_collect(ldg_closure *c, ldg_handle *fine, ldg_handle *coarse, do
   if (c->normalized) return cfail(c, LDG_EINVAL, "the dataset was already normalised; collect before fitting");
   // fine right-hand sides k_f = rhs(u_f) of every instance (plain, no closure)
   for (Group &g : fine->groups) {
-    if (launch_grad(fine, g, g.u)) return cfail(c, LDG_EDEVICE, fine->err);
+    if (launch_grad(fine, g, state_in(g.u))) return cfail(c, LDG_EDEVICE, fine->err);
     StageArgs st = empty_stage();
     st.kout = g.k;
-    if (launch_rhs(fine, g, g.u, st)) return cfail(c, LDG_EDEVICE, fine->err);
+    if (launch_rhs(fine, g, state_in(g.u), st)) return cfail(c, LDG_EDEVICE, fine->err);
   }
   CCHECK(c, hipStreamSynchronize(fine->stream));
-  // project u_f -> coarse u, k_f -> coarse ua (scratch for P k_f)
+  // project u_f -> coarse u, k_f -> coarse acc (FP64 scratch for P k_f)
   for (int i = 0; i < ni; ++i) {
     const Inst &a = fine->inst[i], &b = coarse->inst[i];
     const Group &gf = fine->groups[a.group];
     Group &gc = coarse->groups[b.group];
     std::string err;
     int rc = project_device(coarse->stream, a.p, b.p, a.K, gf.u + inst_offset(fine, i), gc.u + inst_offset(coarse, i), err);
-    if (!rc) rc = project_device(coarse->stream, a.p, b.p, a.K, gf.k + inst_offset(fine, i), gc.ua + inst_offset(coarse, i), err);
+    if (!rc) rc = project_device(coarse->stream, a.p, b.p, a.K, gf.k + inst_offset(fine, i), gc.acc + inst_offset(coarse, i), err);
     if (rc) return cfail(c, rc, err);
   }
   // coarse right-hand side of the projected state, features and targets
   std::mt19937_64 rng(seed);
   for (Group &g : coarse->groups) {
-    if (launch_grad(coarse, g, g.u)) return cfail(c, LDG_EDEVICE, coarse->err);
+    if (launch_grad(coarse, g, state_in(g.u))) return cfail(c, LDG_EDEVICE, coarse->err);
     StageArgs st = empty_stage();
     st.kout = g.k;
-    if (launch_rhs(coarse, g, g.u, st)) return cfail(c, LDG_EDEVICE, coarse->err);
+    if (launch_rhs(coarse, g, state_in(g.u), st)) return cfail(c, LDG_EDEVICE, coarse->err);
     float *Xf = nullptr, *Tf = nullptr;
     long *didx = nullptr;
     if (hipMalloc((void **)&Xf, (size_t)g.K * c->nin * sizeof(float)) != hipSuccess ||
@@ -519,8 +545,8 @@ int ldg_closure_collect(ldg_closure *c, ldg_handle *fine, ldg_handle *coarse, do
     hipError_t e = hipMalloc((void **)&dident, ident.size() * sizeof(float));
     if (e == hipSuccess) e = hipMemcpyAsync(dident, ident.data(), ident.size() * sizeof(float), hipMemcpyHostToDevice, coarse->stream);
     if (e == hipSuccess) {
-      features_group(coarse, g, g.u, dident, c->nin, Xf);
-      hipLaunchKernelGGL(mlp_targets, dim3(grid_for((long)g.nU)), dim3(256), 0, coarse->stream, (const double *)g.ua,
+      features_group(coarse, g, state_in(g.u), dident, c->nin, Xf);
+      hipLaunchKernelGGL(mlp_targets, dim3(grid_for((long)g.nU)), dim3(256), 0, coarse->stream, (const double *)g.acc,
                          (const double *)g.k, (long)g.nU, Tf);
       e = hipGetLastError();
     }

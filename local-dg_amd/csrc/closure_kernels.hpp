@@ -117,8 +117,8 @@ __global__ void __launch_bounds__(256) mlp_gemm(int M, int N, int K, const float
 // Features of elements [0, K) of one instance: X[e][j] = (x_j - mean_j) *
 // istd_j with x = (state minus the background (rho0, 0, 0, 0, E0): nU
 // values, then the gradient: nQ values), both in their per-element layout.
-template <typename TQ>
-__global__ void mlp_features(const double *__restrict__ U, const TQ *__restrict__ Q, long K, int dofs,
+template <typename TQ, typename UT>
+__global__ void mlp_features(const UT *__restrict__ U, const TQ *__restrict__ Q, long K, int dofs,
                              double rho0, double E0, const float *__restrict__ mean, const float *__restrict__ istd,
                              float *__restrict__ X) {
   const int nU = 5 * dofs, nin = 20 * dofs;
@@ -129,7 +129,7 @@ __global__ void mlp_features(const double *__restrict__ U, const TQ *__restrict_
     double x;
     if (j < nU) {
       const int s = j / dofs;
-      x = U[e * nU + j] - (s == 0 ? rho0 : s == 4 ? E0 : 0.0);
+      x = (double)U[e * nU + j] - (s == 0 ? rho0 : s == 4 ? E0 : 0.0);
     } else {
       x = (double)Q[e * (long)(15 * dofs) + (j - nU)];
     }
@@ -149,13 +149,15 @@ __global__ void mlp_targets(const double *__restrict__ Pk, const double *__restr
 // (2 rho_j) caused by a correction c of (rho, m) (one wave; lanes over the
 // nodes; every lane returns the sum).
 // The correction is c = y * ostd + omean (y: the normalised output row).
-__device__ inline double ke_production(const double *__restrict__ u, int dofs, const float *y,
+// u: the element's state, or its fluctuation (then rho_add = rho0).
+template <typename UT>
+__device__ inline double ke_production(const UT *__restrict__ u, double rho_add, int dofs, const float *y,
                                        const float *__restrict__ ostd, const float *__restrict__ omean, int lane) {
   auto c = [&](int k) { return (double)y[k] * (double)ostd[k] + (double)omean[k]; };
   double pr = 0.0;
   for (int j = lane; j < dofs; j += 64) {
-    const double ir = 1.0 / u[j];
-    const double v0 = u[dofs + j] * ir, v1 = u[2 * dofs + j] * ir, v2 = u[3 * dofs + j] * ir;
+    const double ir = 1.0 / ((double)u[j] + rho_add);
+    const double v0 = (double)u[dofs + j] * ir, v1 = (double)u[2 * dofs + j] * ir, v2 = (double)u[3 * dofs + j] * ir;
     pr += v0 * c(dofs + j) + v1 * c(2 * dofs + j) + v2 * c(3 * dofs + j) - 0.5 * (v0 * v0 + v1 * v1 + v2 * v2) * c(j);
   }
 #pragma unroll
@@ -167,10 +169,12 @@ __device__ inline double ke_production(const double *__restrict__ u, int dofs, c
 // corr = Y * ostd + omean, applied (gate = 1) only if the RMS of the
 // normalised output Y is at most `bound` and, with `dissipative`, if it
 // does not raise the element's kinetic energy; else gate = 0 (plain RHS).
+template <typename UT>
 __global__ void __launch_bounds__(64) mlp_correction(const float *__restrict__ Y, long K, int nout,
                                                      const float *__restrict__ omean, const float *__restrict__ ostd,
-                                                     double bound, int dissipative, const double *__restrict__ U,
-                                                     double *__restrict__ corr, unsigned char *__restrict__ gate) {
+                                                     double bound, int dissipative, const UT *__restrict__ U,
+                                                     double rho_add, double *__restrict__ corr,
+                                                     unsigned char *__restrict__ gate) {
   for (long e = blockIdx.x; e < K; e += gridDim.x) {
     const float *y = Y + e * nout;
     float ss = 0.0f;
@@ -178,7 +182,7 @@ __global__ void __launch_bounds__(64) mlp_correction(const float *__restrict__ Y
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) ss += __shfl_xor(ss, o, 64);
     bool on = sqrt((double)ss / nout) <= bound;
-    if (on && dissipative) on = ke_production(U + e * nout, nout / 5, y, ostd, omean, threadIdx.x) <= 0.0;
+    if (on && dissipative) on = ke_production(U + e * nout, rho_add, nout / 5, y, ostd, omean, threadIdx.x) <= 0.0;
     for (int j = threadIdx.x; j < nout; j += 64)
       corr[e * nout + j] = on ? (double)y[j] * (double)ostd[j] + (double)omean[j] : 0.0;
     if (threadIdx.x == 0) gate[e] = on ? 1 : 0;
@@ -350,15 +354,15 @@ __device__ inline void wave_layer(const unsigned short *__restrict__ A, const un
   }
 }
 
-template <typename TQ, int NIN, int HID, int NOUT>
-__global__ void __launch_bounds__(256) mlp_infer(const double *__restrict__ U, const TQ *__restrict__ Q, long K,
+template <typename TQ, typename UT, int NIN, int HID, int NOUT>
+__global__ void __launch_bounds__(256) mlp_infer(const UT *__restrict__ U, const TQ *__restrict__ Q, long K,
                                                  double rho0, double E0, const float *__restrict__ norm,
                                                  const unsigned short *__restrict__ W1b,
                                                  const unsigned short *__restrict__ W2b,
                                                  const unsigned short *__restrict__ W3b, const float *__restrict__ b1,
                                                  const float *__restrict__ b2, const float *__restrict__ b3,
-                                                 double bound, int dissipative, double *__restrict__ corr,
-                                                 unsigned char *__restrict__ gate) {
+                                                 double bound, int dissipative, double rho_add,
+                                                 double *__restrict__ corr, unsigned char *__restrict__ gate) {
   constexpr int DOFS = NIN / 20, NU = 5 * DOFS, NQ = 15 * DOFS;
   constexpr int K1P = (NIN + 15) / 16 * 16, NOUTP = (NOUT + 31) / 32 * 32;
   constexpr int XS = K1P + 8, HS = HID + 8;
@@ -381,13 +385,13 @@ __global__ void __launch_bounds__(256) mlp_infer(const double *__restrict__ U, c
   __syncthreads();
   {
     constexpr int B = 8;
-    const double *Ub = U + e0 * NU;
+    const UT *Ub = U + e0 * NU;
     for (int base = 0; base < FR * NU; base += 256 * B) {
       double v[B];
 #pragma unroll
       for (int i = 0; i < B; ++i) {
         const int idx = base + i * 256 + tid;
-        v[i] = idx < rows * NU ? Ub[idx] : 0.0;
+        v[i] = idx < rows * NU ? (double)Ub[idx] : 0.0;
       }
 #pragma unroll
       for (int i = 0; i < B; ++i) {
@@ -474,7 +478,8 @@ __global__ void __launch_bounds__(256) mlp_infer(const double *__restrict__ U, c
     for (int o = 32; o > 0; o >>= 1) ss += __shfl_xor(ss, o, 64);
     bool on = sqrt((double)ss / NOUT) <= bound;
     const long e = e0 + r;
-    if (on && dissipative) on = ke_production(U + e * NU, DOFS, &Ys[r][0], ostd, omean, lane) <= 0.0;
+    if (on && dissipative)
+      on = ke_production(U + e * NU, rho_add, DOFS, &Ys[r][0], ostd, omean, lane) <= 0.0;
     for (int j = lane; j < NOUT; j += 64)
       corr[e * NOUT + j] = on ? (double)Ys[r][j] * (double)ostd[j] + (double)omean[j] : 0.0;
     if (lane == 0) gate[e] = on ? 1 : 0;

@@ -29,11 +29,11 @@ struct Inst {
   int group, slot;        // group and position inside it
 };
 
-// Per (group, candidate E): instance table and block -> instance map.
+// Per (group, candidate E): instance table and logical workgroup table.
 struct Tiling {
   int E = 1, nblocks = 0;
   InstDesc *d_inst = nullptr;
-  int *d_blk = nullptr;   // nullptr when the group has one instance
+  BlockDesc *d_blk = nullptr;
   std::vector<InstDesc> h_inst;
 };
 
@@ -53,7 +53,8 @@ struct Group {
   std::vector<int> members;   // handle-level instance indices
   long K = 0;
   size_t nU = 0;
-  double *u = nullptr, *acc = nullptr, *ua = nullptr, *ub = nullptr, *k = nullptr;
+  double *u = nullptr, *acc = nullptr, *k = nullptr;
+  void *ua = nullptr, *ub = nullptr;  // stage inputs: FP64 state, or FP32 fluctuations in the mixed path
   void *q = nullptr;
   double *ke_part = nullptr, *diag_part = nullptr, *d_scale = nullptr;
   void *ops = nullptr;
@@ -97,9 +98,18 @@ int fail(ldg_handle *h, int code, const std::string &msg);
       return ::ldg_host::fail((h), LDG_EDEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
+// Input of a stage's kernels: the conserved state (FP64), or in the mixed
+// path the FP32 fluctuation about the background that the previous stage's
+// epilogue wrote.
+struct StageIn {
+  const void *p;
+  bool fluct;
+};
+inline StageIn state_in(const double *u) { return StageIn{u, false}; }
+
 StageArgs empty_stage();
-int launch_grad(ldg_handle *h, Group &g, const double *U, int c = -1);
-int launch_rhs(ldg_handle *h, Group &g, const double *U, const StageArgs &st, int c = -1);
+int launch_grad(ldg_handle *h, Group &g, StageIn U, int c = -1);
+int launch_rhs(ldg_handle *h, Group &g, StageIn U, const StageArgs &st, int c = -1);
 void drop_graph(ldg_handle *h);
 
 // Offset (in doubles) and size of instance i inside its group's arrays.
@@ -109,7 +119,7 @@ size_t inst_values(const ldg_handle *h, int i);
 // closure.hip: before a stage's rhs launch, evaluates the group's closure on
 // the stage input U (and the gradient just computed into g.q) and points st
 // at the correction; no-op without a closure.
-int closure_stage(ldg_handle *h, Group &g, const double *U, StageArgs &st);
+int closure_stage(ldg_handle *h, Group &g, StageIn U, StageArgs &st);
 void closure_release(Group &g);
 
 }  // namespace ldg_host

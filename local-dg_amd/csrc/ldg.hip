@@ -104,42 +104,51 @@ StageArgs empty_stage() {
 // ---------------------------------------------------------------- launches
 
 template <int N, int E>
-void launch_grad_e(ldg_handle *h, Group &g, const Tiling &t, const double *U) {
+void launch_grad_e(ldg_handle *h, Group &g, const Tiling &t, StageIn U) {
   LaunchArgs la{t.d_inst, t.d_blk, t.nblocks, (t.nblocks + 7) / 8};
   const Ops<N, double> *op = (const Ops<N, double> *)g.ops;
-  if (h->mixed)
-    hipLaunchKernelGGL((ldg_grad_kernel<N, E, float>), dim3(8 * la.per_xcd), dim3(NT), 0, h->stream, U,
+  const dim3 grid(8 * la.per_xcd), block(NT);
+  if (h->mixed && U.fluct)
+    hipLaunchKernelGGL((ldg_grad_kernel<N, E, float, float>), grid, block, 0, h->stream, (const float *)U.p,
+                       (float *)g.q, la, op);
+  else if (h->mixed)
+    hipLaunchKernelGGL((ldg_grad_kernel<N, E, float, double>), grid, block, 0, h->stream, (const double *)U.p,
                        (float *)g.q, la, op);
   else
-    hipLaunchKernelGGL((ldg_grad_kernel<N, E, double>), dim3(8 * la.per_xcd), dim3(NT), 0, h->stream, U,
+    hipLaunchKernelGGL((ldg_grad_kernel<N, E, double, double>), grid, block, 0, h->stream, (const double *)U.p,
                        (double *)g.q, la, op);
 }
 
 template <int N, int E>
-void launch_rhs_e(ldg_handle *h, Group &g, const Tiling &t, const double *U, const StageArgs &st) {
+void launch_rhs_e(ldg_handle *h, Group &g, const Tiling &t, StageIn U, const StageArgs &st) {
   LaunchArgs la{t.d_inst, t.d_blk, t.nblocks, (t.nblocks + 7) / 8};
   const Ops<N, double> *op = (const Ops<N, double> *)g.ops;
-  if (h->mixed)
-    hipLaunchKernelGGL((ldg_rhs_kernel<N, E, float>), dim3(8 * la.per_xcd), dim3(NT), 0, h->stream, U,
+  const dim3 grid(8 * la.per_xcd), block(NT);
+  if (h->mixed && U.fluct)
+    hipLaunchKernelGGL((ldg_rhs_kernel<N, E, float, float>), grid, block, 0, h->stream, (const float *)U.p,
+                       (const float *)g.q, la, op, st);
+  else if (h->mixed)
+    hipLaunchKernelGGL((ldg_rhs_kernel<N, E, float, double>), grid, block, 0, h->stream, (const double *)U.p,
                        (const float *)g.q, la, op, st);
   else
-    hipLaunchKernelGGL((ldg_rhs_kernel<N, E, double>), dim3(8 * la.per_xcd), dim3(NT), 0, h->stream, U,
+    hipLaunchKernelGGL((ldg_rhs_kernel<N, E, double, double>), grid, block, 0, h->stream, (const double *)U.p,
                        (const double *)g.q, la, op, st);
 }
 
 template <int N>
-void launch_grad_n(ldg_handle *h, Group &g, int c, const double *U) {
+void launch_grad_n(ldg_handle *h, Group &g, int c, StageIn U) {
   if (c == 0) launch_grad_e<N, Cand<N>::grad[0]>(h, g, g.grad[0], U);
   else launch_grad_e<N, Cand<N>::grad[1]>(h, g, g.grad[1], U);
 }
 
 template <int N>
-void launch_rhs_n(ldg_handle *h, Group &g, int c, const double *U, const StageArgs &st) {
+void launch_rhs_n(ldg_handle *h, Group &g, int c, StageIn U, const StageArgs &st) {
   if (c == 0) launch_rhs_e<N, Cand<N>::rhs[0]>(h, g, g.rhs[0], U, st);
   else launch_rhs_e<N, Cand<N>::rhs[1]>(h, g, g.rhs[1], U, st);
 }
 
-int launch_grad(ldg_handle *h, Group &g, const double *U, int c) {
+int launch_grad(ldg_handle *h, Group &g, StageIn U, int c) {
+  if (U.fluct && !h->mixed) return fail(h, LDG_EINVAL, "FP32 stage input outside the mixed path");
   if (c < 0) c = g.cgrad;
   switch (g.n) {
     case 2: launch_grad_n<2>(h, g, c, U); break;
@@ -153,7 +162,8 @@ int launch_grad(ldg_handle *h, Group &g, const double *U, int c) {
   return LDG_OK;
 }
 
-int launch_rhs(ldg_handle *h, Group &g, const double *U, const StageArgs &st, int c) {
+int launch_rhs(ldg_handle *h, Group &g, StageIn U, const StageArgs &st, int c) {
+  if (U.fluct && !h->mixed) return fail(h, LDG_EINVAL, "FP32 stage input outside the mixed path");
   if (c < 0) c = g.crhs;
   switch (g.n) {
     case 2: launch_rhs_n<2>(h, g, c, U, st); break;
@@ -191,7 +201,7 @@ int upload_ops(ldg_handle *h, Group &g) {
 int build_tiling(ldg_handle *h, Group &g, Tiling &t, int E) {
   t.E = E;
   t.h_inst.clear();
-  std::vector<int> blk;
+  std::vector<BlockDesc> blk;
   long off = 0;
   int boff = 0;
   for (int idx : g.members) {
@@ -217,7 +227,12 @@ int build_tiling(ldg_handle *h, Group &g, Tiling &t, int E) {
     for (int l = 0; l < 3; ++l) d.ialpha[l] = 2.0 / in.dx[l];
     d.dt = 0.0;
     d.ph = phys_args(h->cfg, in);
-    for (int b = 0; b < d.nblocks; ++b) blk.push_back(in.slot);
+    if (in.slot >= (1 << 23)) return fail(h, LDG_EINVAL, "too many instances in one order group");
+    for (int b = 0; b < d.nblocks; ++b) {
+      int ec[3], ne;
+      block_elements(d, E, b, ec, ne);
+      blk.push_back(BlockDesc{ec[0], ec[1], ec[2], in.slot << 8 | ne});
+    }
     off += in.K;
     boff += d.nblocks;
     t.h_inst.push_back(d);
@@ -225,10 +240,8 @@ int build_tiling(ldg_handle *h, Group &g, Tiling &t, int E) {
   t.nblocks = boff;
   HIPCHECK(h, hipMalloc((void **)&t.d_inst, sizeof(InstDesc) * t.h_inst.size()));
   HIPCHECK(h, hipMemcpy(t.d_inst, t.h_inst.data(), sizeof(InstDesc) * t.h_inst.size(), hipMemcpyHostToDevice));
-  if (g.members.size() > 1) {
-    HIPCHECK(h, hipMalloc((void **)&t.d_blk, sizeof(int) * blk.size()));
-    HIPCHECK(h, hipMemcpy(t.d_blk, blk.data(), sizeof(int) * blk.size(), hipMemcpyHostToDevice));
-  }
+  HIPCHECK(h, hipMalloc((void **)&t.d_blk, sizeof(BlockDesc) * blk.size()));
+  HIPCHECK(h, hipMemcpy(t.d_blk, blk.data(), sizeof(BlockDesc) * blk.size(), hipMemcpyHostToDevice));
   return LDG_OK;
 }
 
@@ -299,9 +312,14 @@ int create_impl(const ldg_config *cfg, const ldg_instance *insts, int n, ldg_han
     g.nU = (size_t)g.n * g.n * g.n * NSTATE * g.K;
     g.basis = make_basis(g.n - 1);
     const size_t bytes = g.nU * sizeof(double);
-    double **arrs[5] = {&g.u, &g.acc, &g.ua, &g.ub, &g.k};
+    double **arrs[3] = {&g.u, &g.acc, &g.k};
     for (double **p : arrs)
       if (hipMalloc((void **)p, bytes) != hipSuccess) { h->err = "device allocation failed (state)"; return bad(LDG_ENOMEM); }
+    const size_t sbytes = g.nU * (h->mixed ? sizeof(float) : sizeof(double));
+    if (hipMalloc(&g.ua, sbytes) != hipSuccess || hipMalloc(&g.ub, sbytes) != hipSuccess) {
+      h->err = "device allocation failed (stage inputs)";
+      return bad(LDG_ENOMEM);
+    }
     if (hipMalloc(&g.q, g.nU * 3 * (h->mixed ? sizeof(float) : sizeof(double))) != hipSuccess) {
       h->err = "device allocation failed (gradient)";
       return bad(LDG_ENOMEM);
@@ -361,8 +379,8 @@ int rk4_step(ldg_handle *h, bool ke) {
   static const double a[3] = {0.5, 0.5, 1.0};
   int first = 0;
   for (Group &g : h->groups) {
-    const double *U = g.u;
-    double *bufs[2] = {g.ua, g.ub};
+    StageIn U = state_in(g.u);
+    void *bufs[2] = {g.ua, g.ub};
     for (int s = 0; s < 4; ++s) {
       int rc = launch_grad(h, g, U);
       if (rc) return rc;
@@ -371,13 +389,17 @@ int rk4_step(ldg_handle *h, bool ke) {
       st.acc = g.acc;
       st.acc_init = (s == 0);
       st.cb = b[s];
-      if (s < 3) { st.unext = bufs[s & 1]; st.ca = a[s]; }
+      if (s < 3) {
+        if (h->mixed) st.unext32 = (float *)bufs[s & 1];
+        else st.unext = (double *)bufs[s & 1];
+        st.ca = a[s];
+      }
       if (s == 3 && ke) st.ke_part = g.ke_part;
       rc = closure_stage(h, g, U, st);
       if (rc) return rc;
       rc = launch_rhs(h, g, U, st);
       if (rc) return rc;
-      U = bufs[s & 1];
+      U = StageIn{bufs[s & 1], h->mixed};
     }
     if (ke) {
       const Tiling &t = g.rhs[g.crhs];
@@ -572,8 +594,10 @@ void ldg_destroy(ldg_handle *h) {
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   drop_graph(h);
   for (Group &g : h->groups) {
-    double *ptrs[] = {g.u, g.acc, g.ua, g.ub, g.k, g.ke_part, g.diag_part, g.d_scale};
+    double *ptrs[] = {g.u, g.acc, g.k, g.ke_part, g.diag_part, g.d_scale};
     for (double *p : ptrs) if (p) (void)hipFree(p);
+    if (g.ua) (void)hipFree(g.ua);
+    if (g.ub) (void)hipFree(g.ub);
     if (g.q) (void)hipFree(g.q);
     if (g.ops) (void)hipFree(g.ops);
     for (int c = 0; c < kMaxCand; ++c) { free_tiling(g.rhs[c]); free_tiling(g.grad[c]); }
@@ -676,14 +700,15 @@ int ldg_rhs(ldg_handle *h, const double *u, double *k, double *Du) {
   if (h->inst.size() != 1) return fail(h, LDG_EINVAL, "ldg_rhs needs a single-instance handle");
   Group &g = h->groups[0];
   const size_t bytes = g.nU * sizeof(double);
-  HIPCHECK(h, hipMemcpyAsync(g.ua, u, bytes, hipMemcpyHostToDevice, h->stream));
-  int rc = launch_grad(h, g, g.ua);
+  // the FP64 running-sum array is free between steps: it takes the input
+  HIPCHECK(h, hipMemcpyAsync(g.acc, u, bytes, hipMemcpyHostToDevice, h->stream));
+  int rc = launch_grad(h, g, state_in(g.acc));
   if (rc) return rc;
   StageArgs st = empty_stage();
   st.kout = g.k;
-  rc = closure_stage(h, g, g.ua, st);
+  rc = closure_stage(h, g, state_in(g.acc), st);
   if (rc) return rc;
-  rc = launch_rhs(h, g, g.ua, st);
+  rc = launch_rhs(h, g, state_in(g.acc), st);
   if (rc) return rc;
   HIPCHECK(h, hipMemcpyAsync(k, g.k, bytes, hipMemcpyDeviceToHost, h->stream));
   if (Du) {
@@ -735,16 +760,16 @@ int ldg_instance_diagnostics(ldg_handle *h, int i, double out[3]) {
   // of u; the Gauss-point sums come from the volume phase of the same launch
   int first = 0;
   for (Group &g : h->groups) {
-    int rc = launch_grad(h, g, g.u);
+    int rc = launch_grad(h, g, state_in(g.u));
     if (rc) return rc;
     StageArgs st = empty_stage();
     st.u0 = g.u;
-    st.acc = g.ua;
+    st.acc = g.acc;
     st.acc_init = 1;
     st.cb = 0.0;
     st.ke_part = g.ke_part;
     st.diag_part = g.diag_part;
-    rc = launch_rhs(h, g, g.u, st);
+    rc = launch_rhs(h, g, state_in(g.u), st);
     if (rc) return rc;
     const Tiling &t = g.rhs[g.crhs];
     hipLaunchKernelGGL(ldg_finalize_kernel, dim3(g.members.size()), dim3(NT), 0, h->stream, (const double *)g.ke_part,
@@ -796,11 +821,11 @@ int ldg_autotune(ldg_handle *h, int *chosen) {
           if (rep == 1) HIPCHECK(h, hipEventRecord(e0, h->stream));
           int rc;
           if (kind == 0) {
-            rc = launch_grad(h, g, g.u, c);
+            rc = launch_grad(h, g, state_in(g.u), c);
           } else {
             StageArgs st = empty_stage();
             st.kout = g.k;
-            rc = launch_rhs(h, g, g.u, st, c);
+            rc = launch_rhs(h, g, state_in(g.u), st, c);
           }
           if (rc) return rc;
         }
@@ -836,8 +861,8 @@ int ldg_kernel_times(ldg_handle *h, double dt, int nsteps, double ms[2]) {
   double tot[2] = {0, 0};
   int count = 0;
   for (int it = 0; it < nsteps; ++it) {
-    const double *U = g.u;
-    double *bufs[2] = {g.ua, g.ub};
+    StageIn U = state_in(g.u);
+    void *bufs[2] = {g.ua, g.ub};
     for (int s = 0; s < 4; ++s) {
       HIPCHECK(h, hipEventRecord(ev[0], h->stream));
       rc = launch_grad(h, g, U);
@@ -845,7 +870,11 @@ int ldg_kernel_times(ldg_handle *h, double dt, int nsteps, double ms[2]) {
       HIPCHECK(h, hipEventRecord(ev[1], h->stream));
       StageArgs st = empty_stage();
       st.u0 = g.u; st.acc = g.acc; st.acc_init = (s == 0); st.cb = b[s];
-      if (s < 3) { st.unext = bufs[s & 1]; st.ca = a[s]; }
+      if (s < 3) {
+        if (h->mixed) st.unext32 = (float *)bufs[s & 1];
+        else st.unext = (double *)bufs[s & 1];
+        st.ca = a[s];
+      }
       rc = launch_rhs(h, g, U, st);
       if (rc) return rc;
       HIPCHECK(h, hipEventRecord(ev[2], h->stream));
@@ -854,7 +883,7 @@ int ldg_kernel_times(ldg_handle *h, double dt, int nsteps, double ms[2]) {
       HIPCHECK(h, hipEventElapsedTime(&t0, ev[0], ev[1]));
       HIPCHECK(h, hipEventElapsedTime(&t1, ev[1], ev[2]));
       tot[0] += t0; tot[1] += t1; ++count;
-      U = bufs[s & 1];
+      U = StageIn{bufs[s & 1], h->mixed};
     }
     std::swap(g.u, g.acc);
   }

@@ -62,6 +62,7 @@ struct StageArgs {
   const double *u0;   // u^n (RK base state)
   double *acc;        // running u^n + dt sum b_s k_s
   double *unext;      // next stage input u^n + dt a_s k_s
+  float *unext32;     // or, in the mixed path, that input as FP32 fluctuations about the background
   double *kout;       // raw k (parity / single RHS)
   double cb, ca;      // RK4 coefficients b_s, a_s (times each instance's dt)
   int acc_init;       // 1: acc = u0 + dt b k   0: acc += dt b k
@@ -88,32 +89,37 @@ struct InstDesc {
   PhysArgs ph;
 };
 
+// One logical workgroup of a launch: the instance-local coordinates of its
+// first element and inst_ne = instance << 8 | element count (count 0: a
+// padding block).  Built on the host (block_elements), so that no kernel
+// divides by a run-time value to find its elements or their neighbours.
+struct BlockDesc {
+  int ex, ey, ez, inst_ne;
+};
+
 // Elements of an instance's logical workgroup lbl (0-based within the
 // instance): blocks are E consecutive elements of one x-row, and blocks are
 // visited brick by brick (tb blocks per brick, bricks x-fastest), so that the
 // workgroups resident at the same time on an XCD cover a compact region and
 // their face neighbours in y and z are mostly in that XCD's L2.  Returns the
-// first element (local index) and the count, 0 for a padding block of a
-// partial brick.
-__host__ __device__ inline void block_elements(const InstDesc &d, int E, int lbl, long &kl, int &ne) {
+// coordinates of the first element and the count, 0 for a padding block of
+// a partial brick.
+inline void block_elements(const InstDesc &d, int E, int lbl, int ec[3], int &ne) {
   const int bs = d.tb[0] * d.tb[1] * d.tb[2];
   const int brick = lbl / bs, off = lbl - brick * bs;
   const int Bx = brick % d.nbrick[0], By = (brick / d.nbrick[0]) % d.nbrick[1], Bz = brick / (d.nbrick[0] * d.nbrick[1]);
   const int ox = off % d.tb[0], oy = (off / d.tb[0]) % d.tb[1], oz = off / (d.tb[0] * d.tb[1]);
   const int bx = Bx * d.tb[0] + ox, ey = By * d.tb[1] + oy, ez = Bz * d.tb[2] + oz;
-  if (bx >= d.nbx || ey >= d.ny || ez >= d.nz) {
-    kl = 0;
-    ne = 0;
-    return;
-  }
-  const int ex = bx * E;
-  kl = ex + (long)d.nx * (ey + (long)d.ny * ez);
-  ne = (d.nx - ex) < E ? (d.nx - ex) : E;
+  ec[0] = ec[1] = ec[2] = 0;
+  ne = 0;
+  if (bx >= d.nbx || ey >= d.ny || ez >= d.nz) return;
+  ec[0] = bx * E; ec[1] = ey; ec[2] = ez;
+  ne = (d.nx - ec[0]) < E ? (d.nx - ec[0]) : E;
 }
 
 struct LaunchArgs {
   const InstDesc *inst;
-  const int *blk_inst;   // logical workgroup -> instance (nullptr: one instance)
+  const BlockDesc *blk;  // per logical workgroup
   int nblocks;           // logical workgroups of the launch
   int per_xcd;           // ceil(nblocks / 8)
 };
@@ -128,33 +134,39 @@ __device__ inline int logical_block(const LaunchArgs &la) {
 }
 
 // The instance of logical workgroup lb, its mesh, and the block's first
-// element (local to the instance) and element count.
-template <int E>
-__device__ inline const InstDesc &block_instance(const LaunchArgs &la, int lb, MeshArgs &m, long &kl, int &ne) {
-  const InstDesc &d = la.inst[la.blk_inst ? la.blk_inst[lb] : 0];
+// element (coordinates ec and index kl, local to the instance) and element
+// count.
+__device__ inline const InstDesc &block_instance(const LaunchArgs &la, int lb, MeshArgs &m, int ec[3], long &kl,
+                                                 int &ne) {
+  const BlockDesc b = la.blk[lb];
+  const InstDesc &d = la.inst[b.inst_ne >> 8];
   m.nx = d.nx; m.ny = d.ny; m.nz = d.nz;
   m.nelem = d.nelem;
   m.ialpha[0] = d.ialpha[0]; m.ialpha[1] = d.ialpha[1]; m.ialpha[2] = d.ialpha[2];
-  block_elements(d, E, lb - d.block_off, kl, ne);
+  ec[0] = b.ex; ec[1] = b.ey; ec[2] = b.ez;
+  kl = b.ex + (long)d.nx * (b.ey + (long)d.ny * b.ez);
+  ne = b.inst_ne & 255;
   return d;
 }
 
-__device__ inline long neighbor(const MeshArgs &m, long K, int d, int dir) {
-  long ex = K % m.nx, ey = (K / m.nx) % m.ny, ez = K / ((long)m.nx * m.ny);
-  if (d == 0) ex = (ex + dir + m.nx) % m.nx;
-  else if (d == 1) ey = (ey + dir + m.ny) % m.ny;
-  else ez = (ez + dir + m.nz) % m.nz;
-  return ex + m.nx * (ey + (long)m.ny * ez);
+// c + dir (dir = +-1) on a periodic axis of n elements
+__device__ inline int wrap_step(int c, int dir, int n) {
+  c += dir;
+  return c < 0 ? c + n : (c >= n ? c - n : c);
 }
 
-// Face neighbours of the block's elements, computed once per workgroup
-// (the index arithmetic involves 64-bit division): nb[e][f] for faces
-// f = -x,+x,-y,+y,-z,+z.
-template <int E>
-__device__ inline void fill_neighbors(const MeshArgs &m, long kl, long off, int ne, long (*nb)[6], int tid) {
+// Face neighbours of the block's elements (periodic; the reference builds
+// the same table as eToE, mesh.cpp:201-315, with ghost slots on the faces),
+// computed once per workgroup from the first element's coordinates ec:
+// nb[e][f] for faces f = -x,+x,-y,+y,-z,+z.
+__device__ inline void fill_neighbors(const MeshArgs &m, const int ec[3], long off, int ne, long (*nb)[6], int tid) {
   if (tid < ne * 6) {
-    const int e = tid / 6, f = tid % 6;
-    nb[e][f] = off + neighbor(m, kl + e, f >> 1, (f & 1) ? 1 : -1);
+    const int e = tid / 6, f = tid % 6, dim = f >> 1, dir = (f & 1) ? 1 : -1;
+    int x = ec[0] + e, y = ec[1], z = ec[2];
+    if (dim == 0) x = wrap_step(x, dir, m.nx);
+    else if (dim == 1) y = wrap_step(y, dir, m.ny);
+    else z = wrap_step(z, dir, m.nz);
+    nb[e][f] = off + x + (long)m.nx * (y + (long)m.ny * z);
   }
 }
 
@@ -318,8 +330,8 @@ struct GradWaves {
   static constexpr int value = want < by_lds ? want : by_lds;
 };
 
-template <int N, int E, typename TQ>
-__global__ void __launch_bounds__(NT, (GradWaves<N, E, TQ>::value)) ldg_grad_kernel(const double *__restrict__ U, TQ *__restrict__ Q,
+template <int N, int E, typename TQ, typename UT>
+__global__ void __launch_bounds__(NT, (GradWaves<N, E, TQ>::value)) ldg_grad_kernel(const UT *__restrict__ U, TQ *__restrict__ Q,
                                                         LaunchArgs la, const Ops<N, double> *__restrict__ opp) {
   constexpr int NN = N * N * N, NF = N * N;
   __shared__ double su[E][NSTATE][NN];
@@ -330,33 +342,35 @@ __global__ void __launch_bounds__(NT, (GradWaves<N, E, TQ>::value)) ldg_grad_ker
   if (lb < 0) return;
   MeshArgs m;
   long kl;
-  int ne;
-  const InstDesc &d = block_instance<E>(la, lb, m, kl, ne);
+  int ne, ec[3];
+  const InstDesc &d = block_instance(la, lb, m, ec, kl, ne);
   if (ne == 0) return;                // padding block of a partial brick
   const long k0 = d.elem_off + kl;    // first element of the block in the arrays
   const int tid = threadIdx.x;
-  fill_neighbors<E>(m, kl, d.elem_off, ne, snb, tid);
+  fill_neighbors(m, ec, d.elem_off, ne, snb, tid);
 
   if (tid < N * (N + 1)) {
     const int i = tid / (N + 1), j = tid % (N + 1);
     sD[i][j] = j < N ? opp->Dn[i][j] : opp->L0[i];
   }
   // the block's state (contiguous) into LDS: all of a thread's loads are
-  // issued before the first is stored
+  // issued before the first is stored.  The loads are unconditional (an
+  // index past the block is clamped to its last word) and converted only
+  // when stored: a load under a branch is waited for at the branch's end.
   {
     constexpr int UL = (E * NSTATE * NN + NT - 1) / NT;
-    const double *Ub = U + k0 * NSTATE * NN;
+    const UT *Ub = U + k0 * NSTATE * NN;
     const int tot = ne * NSTATE * NN;
-    double v[UL];
+    UT v[UL];
 #pragma unroll
     for (int it = 0; it < UL; ++it) {
       const int w = tid + it * NT;
-      v[it] = w < tot ? Ub[w] : 0.0;
+      v[it] = Ub[w < tot ? w : tot - 1];
     }
 #pragma unroll
     for (int it = 0; it < UL; ++it) {
       const int w = tid + it * NT;
-      if (w < tot) (&su[0][0][0])[w] = v[it];
+      if (w < tot) (&su[0][0][0])[w] = (double)v[it];
     }
   }
   __syncthreads();
@@ -365,35 +379,26 @@ __global__ void __launch_bounds__(NT, (GradWaves<N, E, TQ>::value)) ldg_grad_ker
   // neighbour's trace on its positive face (neighbour loads batched too)
   {
     constexpr int JL = (E * 3 * NSTATE * NF + NT - 1) / NT;
+    // item w = ((e * 3 + d) * NSTATE + s) * NF + ab is also its index in sj
     const int tot = ne * 3 * NSTATE * NF;
-    double v[JL];
+    UT v[JL];
+    int own[JL];          // the own trace's index in su
 #pragma unroll
     for (int it = 0; it < JL; ++it) {
-      const int w = tid + it * NT;
-      v[it] = 0.0;
-      if (w < tot) {
-        const int e = w / (3 * NSTATE * NF);
-        const int r = w - e * (3 * NSTATE * NF);
-        const int d = r / (NSTATE * NF);
-        const int s = (r / NF) % NSTATE;
-        const int ab = r % NF;
-        const long Kn = snb[e][2 * d];
-        const int base = line_base<N>(d, ab % N, ab / N), st = line_stride<N>(d);
-        v[it] = U[(Kn * NSTATE + s) * NN + base + (N - 1) * st];
-      }
+      const int w = tid + it * NT < tot ? tid + it * NT : tot - 1;
+      const int e = w / (3 * NSTATE * NF);
+      const int r = w - e * (3 * NSTATE * NF);
+      const int d = r / (NSTATE * NF);
+      const int s = (r / NF) % NSTATE;
+      const int ab = r % NF;
+      const int base = line_base<N>(d, ab % N, ab / N), st = line_stride<N>(d);
+      v[it] = U[snb[e][2 * d] * (NSTATE * NN) + (s * NN + base + (N - 1) * st)];
+      own[it] = (e * NSTATE + s) * NN + base;
     }
 #pragma unroll
     for (int it = 0; it < JL; ++it) {
       const int w = tid + it * NT;
-      if (w < tot) {
-        const int e = w / (3 * NSTATE * NF);
-        const int r = w - e * (3 * NSTATE * NF);
-        const int d = r / (NSTATE * NF);
-        const int s = (r / NF) % NSTATE;
-        const int ab = r % NF;
-        const int base = line_base<N>(d, ab % N, ab / N);
-        sj[e][d][s][ab] = su[e][s][base] - v[it];
-      }
+      if (w < tot) (&sj[0][0][0][0])[w] = (&su[0][0][0])[own[it]] - (double)v[it];
     }
   }
   __syncthreads();
@@ -481,11 +486,11 @@ struct Mat {
 // gradient trace chosen by the one-sided switch), interpolate to the face
 // Gauss points, evaluate Roe + viscous fluxes, lift them back to the face
 // nodes and subtract them from the residual sr.
-template <int N, int E, typename T, int D>
-__device__ inline void rhs_face_dir(const double *__restrict__ U, const T *__restrict__ Q, const MeshArgs &m,
+template <int N, int E, typename T, int D, typename UT>
+__device__ inline void rhs_face_dir(const UT *__restrict__ U, const T *__restrict__ Q, const MeshArgs &m,
                                     const SOps<N, T> *__restrict__ so, const Bg<T> &bg,
                                     T (*__restrict__ sbuf)[RhsLds<N, T>::BUF], T (*__restrict__ sr)[NSTATE][N * N * N],
-                                    const long (*nb)[6], long k0, int ne, int tid) {
+                                    const long (*nb)[6], long k0, int ne, int tid, double brho, double bE) {
   constexpr int NN = N * N * N, NF = N * N, NFF = RhsLds<N, T>::NFF;
   constexpr int st_d = D == 0 ? 1 : D == 1 ? N : N * N;
   constexpr int ast = D == 0 ? N : 1;          // memory stride of the face coordinate a
@@ -510,11 +515,11 @@ __device__ inline void rhs_face_dir(const double *__restrict__ U, const T *__res
         const int lb = line_base<N>(D, 0, b);
         if (f < 2 * NSTATE) {
           const int s = f < NSTATE ? f : f - NSTATE;
-          const double *src = f < NSTATE ? U + (K * NSTATE + s) * NN + own + lb
-                                         : U + (nb[e][2 * D + side] * NSTATE + s) * NN + opp + lb;
-          const double bgs = s == 0 ? (double)bg.rho0 : s == 4 ? (double)bg.E0 : 0.0;
+          const UT *src = f < NSTATE ? U + (K * NSTATE + s) * NN + own + lb
+                                     : U + (nb[e][2 * D + side] * NSTATE + s) * NN + opp + lb;
+          const double bgs = s == 0 ? brho : s == 4 ? bE : 0.0;
 #pragma unroll
-          for (int a = 0; a < N; ++a) v[it][a] = T(src[a * ast] - bgs);
+          for (int a = 0; a < N; ++a) v[it][a] = T((double)src[a * ast] - bgs);
         } else {
           const long Ke = side == 0 ? K : nb[e][2 * D + 1];     // own / neighbour's i_d = 0 slice
           const T *src = Q + (Ke * NGRAD + (f - 2 * NSTATE)) * NN + lb;
@@ -633,9 +638,9 @@ __device__ inline void rhs_face_dir(const double *__restrict__ U, const T *__res
   __syncthreads();
 }
 
-template <int N, int E, typename T>
+template <int N, int E, typename T, typename UT>
 __global__ void __launch_bounds__(NT, (RhsLds<N, T>::WAVES))
-ldg_rhs_kernel(const double *__restrict__ U, const T *__restrict__ Q, LaunchArgs la,
+ldg_rhs_kernel(const UT *__restrict__ U, const T *__restrict__ Q, LaunchArgs la,
                const Ops<N, double> *__restrict__ opp, StageArgs st) {
   using L = RhsLds<N, T>;
   constexpr int NN = L::NN, NF = L::NF, NV = L::NV, BUF = L::BUF;
@@ -648,8 +653,8 @@ ldg_rhs_kernel(const double *__restrict__ U, const T *__restrict__ Q, LaunchArgs
   if (lb < 0) return;
   MeshArgs m;
   long kl;
-  int ne;
-  const InstDesc &d = block_instance<E>(la, lb, m, kl, ne);
+  int ne, ec[3];
+  const InstDesc &d = block_instance(la, lb, m, ec, kl, ne);
   if (ne == 0) {                      // padding block of a partial brick: empty partial sums
     if (threadIdx.x == 0) {
       if (st.ke_part) st.ke_part[lb] = 0.0;
@@ -659,6 +664,11 @@ ldg_rhs_kernel(const double *__restrict__ U, const T *__restrict__ Q, LaunchArgs
   }
   const long k0 = d.elem_off + kl;    // first element of the block in the arrays
   const PhysArgs ph = d.ph;
+  // the stage input is the conserved state (UT = double) or its FP32
+  // fluctuation about the background (UT = float, written by the previous
+  // stage's epilogue)
+  constexpr bool FLUCT = sizeof(UT) == 4;
+  const double brho = FLUCT ? 0.0 : ph.rho0, bE = FLUCT ? 0.0 : ph.E0;
   const double dtb = d.dt * st.cb, dta = d.dt * st.ca;
   const int tid = threadIdx.x;
   const SOps<N, T> *so = &sops;
@@ -667,15 +677,15 @@ ldg_rhs_kernel(const double *__restrict__ U, const T *__restrict__ Q, LaunchArgs
   // sweeps and the epilogue
   auto md = [&](int e) -> double * { return reinterpret_cast<double *>(&sbuf[e][0]); };
 
-  fill_neighbors<E>(m, kl, d.elem_off, ne, snb, tid);
+  fill_neighbors(m, ec, d.elem_off, ne, snb, tid);
   load_sops<N, T>(&sops, opp, tid);
   for (int w = tid; w < ne * NSTATE * NN; w += NT) (&sr[0][0][0])[w] = T(0);
   __syncthreads();
 
   // ================= faces, one direction at a time =================
-  rhs_face_dir<N, E, T, 0>(U, Q, m, so, bg, sbuf, sr, snb, k0, ne, tid);
-  rhs_face_dir<N, E, T, 1>(U, Q, m, so, bg, sbuf, sr, snb, k0, ne, tid);
-  rhs_face_dir<N, E, T, 2>(U, Q, m, so, bg, sbuf, sr, snb, k0, ne, tid);
+  rhs_face_dir<N, E, T, 0, UT>(U, Q, m, so, bg, sbuf, sr, snb, k0, ne, tid, brho, bE);
+  rhs_face_dir<N, E, T, 1, UT>(U, Q, m, so, bg, sbuf, sr, snb, k0, ne, tid, brho, bE);
+  rhs_face_dir<N, E, T, 2, UT>(U, Q, m, so, bg, sbuf, sr, snb, k0, ne, tid, brho, bE);
 
   // ================= volume =================
   // load U (minus the background) and Q x-lines from HBM, all loads of the
@@ -694,10 +704,10 @@ ldg_rhs_kernel(const double *__restrict__ U, const T *__restrict__ Q, LaunchArgs
         const int yz = r % NF;
         const long K = k0 + e;
         if (f < NSTATE) {
-          const double *src = U + (K * NSTATE + f) * NN + N * yz;
-          const double bgs = f == 0 ? ph.rho0 : f == 4 ? ph.E0 : 0.0;
+          const UT *src = U + (K * NSTATE + f) * NN + N * yz;
+          const double bgs = f == 0 ? brho : f == 4 ? bE : 0.0;
 #pragma unroll
-          for (int i = 0; i < N; ++i) v[it][i] = T(src[i] - bgs);
+          for (int i = 0; i < N; ++i) v[it][i] = T((double)src[i] - bgs);
         } else {
           const T *src = Q + (K * NGRAD + (f - NSTATE)) * NN + N * yz;
 #pragma unroll
@@ -924,7 +934,7 @@ ldg_rhs_kernel(const double *__restrict__ U, const T *__restrict__ Q, LaunchArgs
   // latency is covered by the inverse-mass sweeps below (1.5 % faster at
   // 64^3, p = 4).  The FP32 layout has room for one of the two only, and
   // there the copies cost more than they save: its epilogue loads directly.
-  const bool need_u0 = st.acc || st.unext;
+  const bool need_u0 = st.acc || st.unext || st.unext32;
   const bool need_acc = st.acc && !st.acc_init;
   constexpr bool PF = sizeof(T) == sizeof(double);
   auto pf_u0 = [&](int e) -> double * { return md(e) + NSTATE * NN; };
@@ -1020,6 +1030,10 @@ ldg_rhs_kernel(const double *__restrict__ U, const T *__restrict__ Q, LaunchArgs
           if (st.ke_part) md(e)[r] = a;
         }
         if (st.unext) st.unext[gi] = ru[it] + dta * k;
+        if (st.unext32) {
+          const int sf = r / NN;
+          st.unext32[gi] = (float)(ru[it] + dta * k - (sf == 0 ? ph.rho0 : sf == 4 ? ph.E0 : 0.0));
+        }
       }
     }
   }
