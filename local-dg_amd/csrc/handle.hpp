@@ -29,11 +29,11 @@ struct Inst {
   int group, slot;        // group and position inside it
 };
 
-// Per (group, candidate E): instance table and logical workgroup table.
+// Per (group, candidate E): instance table and block -> instance map.
 struct Tiling {
   int E = 1, nblocks = 0;
   InstDesc *d_inst = nullptr;
-  BlockDesc *d_blk = nullptr;
+  int *d_blk = nullptr;   // nullptr when the group has one instance
   std::vector<InstDesc> h_inst;
 };
 

@@ -201,7 +201,7 @@ int upload_ops(ldg_handle *h, Group &g) {
 int build_tiling(ldg_handle *h, Group &g, Tiling &t, int E) {
   t.E = E;
   t.h_inst.clear();
-  std::vector<BlockDesc> blk;
+  std::vector<int> blk;
   long off = 0;
   int boff = 0;
   for (int idx : g.members) {
@@ -227,12 +227,12 @@ int build_tiling(ldg_handle *h, Group &g, Tiling &t, int E) {
     for (int l = 0; l < 3; ++l) d.ialpha[l] = 2.0 / in.dx[l];
     d.dt = 0.0;
     d.ph = phys_args(h->cfg, in);
-    if (in.slot >= (1 << 23)) return fail(h, LDG_EINVAL, "too many instances in one order group");
-    for (int b = 0; b < d.nblocks; ++b) {
-      int ec[3], ne;
-      block_elements(d, E, b, ec, ne);
-      blk.push_back(BlockDesc{ec[0], ec[1], ec[2], in.slot << 8 | ne});
-    }
+    d.div_bs = make_fastdiv(d.tb[0] * d.tb[1] * d.tb[2]);
+    d.div_nb0 = make_fastdiv(d.nbrick[0]);
+    d.div_nb1 = make_fastdiv(d.nbrick[1]);
+    d.div_tb0 = make_fastdiv(d.tb[0]);
+    d.div_tb1 = make_fastdiv(d.tb[1]);
+    for (int b = 0; b < d.nblocks; ++b) blk.push_back(in.slot);
     off += in.K;
     boff += d.nblocks;
     t.h_inst.push_back(d);
@@ -240,8 +240,10 @@ int build_tiling(ldg_handle *h, Group &g, Tiling &t, int E) {
   t.nblocks = boff;
   HIPCHECK(h, hipMalloc((void **)&t.d_inst, sizeof(InstDesc) * t.h_inst.size()));
   HIPCHECK(h, hipMemcpy(t.d_inst, t.h_inst.data(), sizeof(InstDesc) * t.h_inst.size(), hipMemcpyHostToDevice));
-  HIPCHECK(h, hipMalloc((void **)&t.d_blk, sizeof(BlockDesc) * blk.size()));
-  HIPCHECK(h, hipMemcpy(t.d_blk, blk.data(), sizeof(BlockDesc) * blk.size(), hipMemcpyHostToDevice));
+  if (g.members.size() > 1) {
+    HIPCHECK(h, hipMalloc((void **)&t.d_blk, sizeof(int) * blk.size()));
+    HIPCHECK(h, hipMemcpy(t.d_blk, blk.data(), sizeof(int) * blk.size(), hipMemcpyHostToDevice));
+  }
   return LDG_OK;
 }
 

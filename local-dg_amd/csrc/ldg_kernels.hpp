@@ -75,6 +75,29 @@ struct StageArgs {
 // One solver instance inside a launch.  A launch covers the elements of all
 // instances of one order (an ensemble); a single solver is a launch of one
 // instance.  Instances are stored back to back in the launch's arrays.
+// n / d for 0 <= n < 2^31 by one 32x32->64-bit multiply and a shift
+// (Granlund and Montgomery, "Division by invariant integers using
+// multiplication", 1994): with l = ceil(log2 d), s = 31 + l and
+// m = ceil(2^s / d) < 2^32, floor(n m / 2^s) = floor(n / d) because the
+// error n (m d - 2^s) / (d 2^s) < n / 2^s <= 2^-l <= 1/d.  Built on the host;
+// with uniform operands the device evaluates it on the scalar unit, where a
+// plain division by a run-time value would take a dozen vector instructions.
+struct FastDiv {
+  unsigned m;
+  int s;
+  __host__ __device__ inline int div(int n) const {
+    return (int)(((unsigned long long)(unsigned)n * m) >> s);
+  }
+};
+inline FastDiv make_fastdiv(int d) {
+  int l = 0;
+  while ((1LL << l) < d) ++l;
+  FastDiv f;
+  f.s = 31 + l;
+  f.m = (unsigned)(((1ULL << f.s) + (unsigned long long)d - 1) / (unsigned long long)d);
+  return f;
+}
+
 struct InstDesc {
   int nx, ny, nz;
   int nblocks;           // its logical workgroups (whole bricks, see block_elements)
@@ -84,17 +107,10 @@ struct InstDesc {
   int nbx;               // element blocks per x-row: ceil(nx / E)
   int tb[3];             // brick of tb[0] x tb[1] x tb[2] blocks (x blocks, y rows, z planes)
   int nbrick[3];         // bricks per direction
+  FastDiv div_bs, div_nb0, div_nb1, div_tb0, div_tb1;   // by tb0 tb1 tb2, nbrick[0], nbrick[1], tb[0], tb[1]
   double ialpha[3];      // 2 / dx_l
   double dt;             // its time step
   PhysArgs ph;
-};
-
-// One logical workgroup of a launch: the instance-local coordinates of its
-// first element and inst_ne = instance << 8 | element count (count 0: a
-// padding block).  Built on the host (block_elements), so that no kernel
-// divides by a run-time value to find its elements or their neighbours.
-struct BlockDesc {
-  int ex, ey, ez, inst_ne;
 };
 
 // Elements of an instance's logical workgroup lbl (0-based within the
@@ -104,11 +120,13 @@ struct BlockDesc {
 // their face neighbours in y and z are mostly in that XCD's L2.  Returns the
 // coordinates of the first element and the count, 0 for a padding block of
 // a partial brick.
-inline void block_elements(const InstDesc &d, int E, int lbl, int ec[3], int &ne) {
+__host__ __device__ inline void block_elements(const InstDesc &d, int E, int lbl, int ec[3], int &ne) {
   const int bs = d.tb[0] * d.tb[1] * d.tb[2];
-  const int brick = lbl / bs, off = lbl - brick * bs;
-  const int Bx = brick % d.nbrick[0], By = (brick / d.nbrick[0]) % d.nbrick[1], Bz = brick / (d.nbrick[0] * d.nbrick[1]);
-  const int ox = off % d.tb[0], oy = (off / d.tb[0]) % d.tb[1], oz = off / (d.tb[0] * d.tb[1]);
+  const int brick = d.div_bs.div(lbl), off = lbl - brick * bs;
+  const int t = d.div_nb0.div(brick), Bx = brick - t * d.nbrick[0];
+  const int Bz = d.div_nb1.div(t), By = t - Bz * d.nbrick[1];
+  const int u = d.div_tb0.div(off), ox = off - u * d.tb[0];
+  const int oz = d.div_tb1.div(u), oy = u - oz * d.tb[1];
   const int bx = Bx * d.tb[0] + ox, ey = By * d.tb[1] + oy, ez = Bz * d.tb[2] + oz;
   ec[0] = ec[1] = ec[2] = 0;
   ne = 0;
@@ -119,7 +137,7 @@ inline void block_elements(const InstDesc &d, int E, int lbl, int ec[3], int &ne
 
 struct LaunchArgs {
   const InstDesc *inst;
-  const BlockDesc *blk;  // per logical workgroup
+  const int *blk_inst;   // logical workgroup -> instance (nullptr: one instance)
   int nblocks;           // logical workgroups of the launch
   int per_xcd;           // ceil(nblocks / 8)
 };
@@ -136,16 +154,15 @@ __device__ inline int logical_block(const LaunchArgs &la) {
 // The instance of logical workgroup lb, its mesh, and the block's first
 // element (coordinates ec and index kl, local to the instance) and element
 // count.
+template <int E>
 __device__ inline const InstDesc &block_instance(const LaunchArgs &la, int lb, MeshArgs &m, int ec[3], long &kl,
                                                  int &ne) {
-  const BlockDesc b = la.blk[lb];
-  const InstDesc &d = la.inst[b.inst_ne >> 8];
+  const InstDesc &d = la.inst[la.blk_inst ? la.blk_inst[lb] : 0];
   m.nx = d.nx; m.ny = d.ny; m.nz = d.nz;
   m.nelem = d.nelem;
   m.ialpha[0] = d.ialpha[0]; m.ialpha[1] = d.ialpha[1]; m.ialpha[2] = d.ialpha[2];
-  ec[0] = b.ex; ec[1] = b.ey; ec[2] = b.ez;
-  kl = b.ex + (long)d.nx * (b.ey + (long)d.ny * b.ez);
-  ne = b.inst_ne & 255;
+  block_elements(d, E, lb - d.block_off, ec, ne);
+  kl = ec[0] + (long)d.nx * (ec[1] + (long)d.ny * ec[2]);
   return d;
 }
 
@@ -343,7 +360,7 @@ __global__ void __launch_bounds__(NT, (GradWaves<N, E, TQ>::value)) ldg_grad_ker
   MeshArgs m;
   long kl;
   int ne, ec[3];
-  const InstDesc &d = block_instance(la, lb, m, ec, kl, ne);
+  const InstDesc &d = block_instance<E>(la, lb, m, ec, kl, ne);
   if (ne == 0) return;                // padding block of a partial brick
   const long k0 = d.elem_off + kl;    // first element of the block in the arrays
   const int tid = threadIdx.x;
@@ -481,48 +498,101 @@ struct Mat {
   }
 };
 
+// Per-thread work items of the face phases.  They are the same for the three
+// directions, so for the larger orders (a few items per thread) they are
+// decomposed once per workgroup (constant divisions cost several integer
+// instructions each) and kept packed in registers; with many small items
+// per thread (p <= 3, many elements per workgroup) they are decomposed where
+// used, which needs fewer registers.  Packing:
+//   g  gather / interpolation lines: e | side << 5 | isQ << 6 | k << 7 | b << 12,
+//      k = trace field (0..4 own U, 5..9 neighbour U) or gradient field
+//      (0..14); the U lines of all elements come first, so that most waves
+//      take one branch of the gather
+//   l  lift lines: e | side << 5 | s << 6 | c << 9
+//   f  face points: e | side << 5 | ab << 6
+// and -1 for an item past the block's elements.
+template <int N, int E>
+struct FaceItems {
+  static constexpr int NU = 2 * NSTATE;                     // trace fields
+  static constexpr int PU = 2 * NU * N, PQ = 2 * NGRAD * N;  // lines per element
+  static constexpr int GL = (E * (PU + PQ) + NT - 1) / NT;
+  static constexpr int LL = (E * 2 * NSTATE * N + NT - 1) / NT;
+  static constexpr int FL = (E * 2 * N * N + NT - 1) / NT;
+  static constexpr bool KEEP = GL <= 2;
+  int ne, tid;
+  int gk[KEEP ? GL : 1], lk[KEEP ? LL : 1], fk[KEEP ? FL : 1];
+
+  __device__ static inline int gcode(int w, int ne) {
+    if (w < E * PU) {
+      const int e = w / PU, r = w - e * PU;
+      return e < ne ? e | (r / (NU * N)) << 5 | ((r / N) % NU) << 7 | (r % N) << 12 : -1;
+    }
+    if (w < E * (PU + PQ)) {
+      const int w2 = w - E * PU, e = w2 / PQ, r = w2 - e * PQ;
+      return e < ne ? e | (r / (NGRAD * N)) << 5 | 1 << 6 | ((r / N) % NGRAD) << 7 | (r % N) << 12 : -1;
+    }
+    return -1;
+  }
+  __device__ static inline int lcode(int w, int ne) {
+    const int e = w / (2 * NSTATE * N), r = w - e * (2 * NSTATE * N);
+    return e < ne ? e | (r / (NSTATE * N)) << 5 | ((r / N) % NSTATE) << 6 | (r % N) << 9 : -1;
+  }
+  __device__ static inline int fcode(int w, int ne) {
+    const int e = w / (2 * N * N), r = w - e * (2 * N * N);
+    return e < ne ? e | (r / (N * N)) << 5 | (r % (N * N)) << 6 : -1;
+  }
+  __device__ inline FaceItems(int ne_, int tid_) : ne(ne_), tid(tid_) {
+    if (KEEP) {
+#pragma unroll
+      for (int it = 0; it < GL; ++it) gk[it] = gcode(tid + it * NT, ne);
+#pragma unroll
+      for (int it = 0; it < LL; ++it) lk[it] = lcode(tid + it * NT, ne);
+#pragma unroll
+      for (int it = 0; it < FL; ++it) fk[it] = fcode(tid + it * NT, ne);
+    }
+  }
+  __device__ inline int g(int it) const { return KEEP ? gk[it] : gcode(tid + it * NT, ne); }
+  __device__ inline int l(int it) const { return KEEP ? lk[it] : lcode(tid + it * NT, ne); }
+  __device__ inline int f(int it) const { return KEEP ? fk[it] : fcode(tid + it * NT, ne); }
+};
+
 // Face terms of one direction D (both faces of every element in the block):
 // gather nodal face data from HBM (own and neighbour traces, and the
 // gradient trace chosen by the one-sided switch), interpolate to the face
 // Gauss points, evaluate Roe + viscous fluxes, lift them back to the face
-// nodes and subtract them from the residual sr.
+// nodes and subtract them from the residual sr.  pU / pQ: per element, its
+// own (0) and its face neighbours' (1 + face) state and gradient.
 template <int N, int E, typename T, int D, typename UT>
-__device__ inline void rhs_face_dir(const UT *__restrict__ U, const T *__restrict__ Q, const MeshArgs &m,
-                                    const SOps<N, T> *__restrict__ so, const Bg<T> &bg,
+__device__ inline void rhs_face_dir(const UT *const (*pU)[7], const T *const (*pQ)[7], const FaceItems<N, E> &fi,
+                                    const MeshArgs &m, const SOps<N, T> *__restrict__ so, const Bg<T> &bg,
                                     T (*__restrict__ sbuf)[RhsLds<N, T>::BUF], T (*__restrict__ sr)[NSTATE][N * N * N],
-                                    const long (*nb)[6], long k0, int ne, int tid, double brho, double bE) {
+                                    double brho, double bE) {
+  using FI = FaceItems<N, E>;
   constexpr int NN = N * N * N, NF = N * N, NFF = RhsLds<N, T>::NFF;
   constexpr int st_d = D == 0 ? 1 : D == 1 ? N : N * N;
   constexpr int ast = D == 0 ? N : 1;          // memory stride of the face coordinate a
+  constexpr int bst = D == 2 ? N : N * N;      // memory stride of the face coordinate b
   // gather nodal face data straight from HBM (all loads first) and
   // interpolate along a: fb[side][f][qa + N b]; f = uK(0..4), uN(5..9), q~(10..24)
   {
-    constexpr int GL = (E * 2 * NFF * N + NT - 1) / NT;
-    const int tot = ne * 2 * NFF * N;
-    T v[GL][N];
+    T v[FI::GL][N];
 #pragma unroll
-    for (int it = 0; it < GL; ++it) {
-      const int w = tid + it * NT;
-      if (w < tot) {
-        const int e = w / (2 * NFF * N);
-        const int r = w - e * (2 * NFF * N);
-        const int side = r / (NFF * N);
-        const int f = (r / N) % NFF;
-        const int b = r % N;
-        const long K = k0 + e;
-        const int own = side ? (N - 1) * st_d : 0;
-        const int opp = side ? 0 : (N - 1) * st_d;
-        const int lb = line_base<N>(D, 0, b);
-        if (f < 2 * NSTATE) {
-          const int s = f < NSTATE ? f : f - NSTATE;
-          const UT *src = f < NSTATE ? U + (K * NSTATE + s) * NN + own + lb
-                                     : U + (nb[e][2 * D + side] * NSTATE + s) * NN + opp + lb;
+    for (int it = 0; it < FI::GL; ++it) {
+      const int gi = fi.g(it);
+      if (gi >= 0) {
+        const int e = gi & 31, side = (gi >> 5) & 1, k = (gi >> 7) & 31, b = gi >> 12;
+        if (!(gi & 64)) {
+          // own trace on this face, or the neighbour's on the opposite face
+          const bool nbr = k >= NSTATE;
+          const int s = nbr ? k - NSTATE : k;
+          const int at = (side ^ nbr) ? (N - 1) * st_d : 0;
+          const UT *src = pU[e][nbr ? 1 + 2 * D + side : 0] + (s * NN + at + b * bst);
           const double bgs = s == 0 ? brho : s == 4 ? bE : 0.0;
 #pragma unroll
           for (int a = 0; a < N; ++a) v[it][a] = T((double)src[a * ast] - bgs);
         } else {
-          const long Ke = side == 0 ? K : nb[e][2 * D + 1];     // own / neighbour's i_d = 0 slice
-          const T *src = Q + (Ke * NGRAD + (f - 2 * NSTATE)) * NN + lb;
+          // own i_d = 0 slice (side 0) or the +d neighbour's (side 1)
+          const T *src = pQ[e][side ? 2 + 2 * D : 0] + (k * NN + b * bst);
 #pragma unroll
           for (int a = 0; a < N; ++a) v[it][a] = src[a * ast];
         }
@@ -531,12 +601,12 @@ __device__ inline void rhs_face_dir(const UT *__restrict__ U, const T *__restric
     Mat<N, T> I;
     I.load(so->I);
 #pragma unroll
-    for (int it = 0; it < GL; ++it) {
-      const int w = tid + it * NT;
-      if (w < tot) {
-        const int e = w / (2 * NFF * N);
-        const int r = w - e * (2 * NFF * N);
-        T *dst = &sbuf[e][(r / N) * NF + N * (r % N)];
+    for (int it = 0; it < FI::GL; ++it) {
+      const int gi = fi.g(it);
+      if (gi >= 0) {
+        const int e = gi & 31, side = (gi >> 5) & 1, k = (gi >> 7) & 31, b = gi >> 12;
+        const int slot = side * NFF + ((gi & 64) ? FI::NU + k : k);
+        T *dst = &sbuf[e][slot * NF + N * b];
 #pragma unroll
         for (int q = 0; q < N; ++q) {
           T acc = T(0);
@@ -552,61 +622,66 @@ __device__ inline void rhs_face_dir(const UT *__restrict__ U, const T *__restric
   {
     Mat<N, T> I;
     I.load(so->I);
-    for (int w = tid; w < ne * 2 * NFF * N; w += NT) {
-      const int e = w / (2 * NFF * N);
-      const int r = w - e * (2 * NFF * N);
-      T *p = &sbuf[e][(r / N) * NF + r % N];
-      T v[N];
 #pragma unroll
-      for (int b = 0; b < N; ++b) v[b] = p[b * N];
+    for (int it = 0; it < FI::GL; ++it) {
+      const int gi = fi.g(it);
+      if (gi >= 0) {
+        const int e = gi & 31, side = (gi >> 5) & 1, k = (gi >> 7) & 31, qa = gi >> 12;
+        const int slot = side * NFF + ((gi & 64) ? FI::NU + k : k);
+        T *p = &sbuf[e][slot * NF + qa];
+        T v[N];
 #pragma unroll
-      for (int q = 0; q < N; ++q) {
-        T acc = T(0);
+        for (int b = 0; b < N; ++b) v[b] = p[b * N];
 #pragma unroll
-        for (int b = 0; b < N; ++b) acc += I.v[q][b] * v[b];
-        p[q * N] = acc;
+        for (int q = 0; q < N; ++q) {
+          T acc = T(0);
+#pragma unroll
+          for (int b = 0; b < N; ++b) acc += I.v[q][b] * v[b];
+          p[q * N] = acc;
+        }
       }
     }
   }
   __syncthreads();
   // flux at each face Gauss point, weighted for the lift
-  for (int w = tid; w < ne * 2 * NF; w += NT) {
-    const int e = w / (2 * NF);
-    const int r = w - e * (2 * NF);
-    const int side = r / NF;
-    const int ab = r % NF;
-    T *fb = &sbuf[e][side * NFF * NF + ab];
-    T uK[5], uN[5], qq[15], out[5];
 #pragma unroll
-    for (int s = 0; s < 5; ++s) { uK[s] = fb[s * NF]; uN[s] = fb[(5 + s) * NF]; }
+  for (int it = 0; it < FI::FL; ++it) {
+    const int fp = fi.f(it);
+    if (fp >= 0) {
+      const int e = fp & 31, side = (fp >> 5) & 1, ab = fp >> 6;
+      T *fb = &sbuf[e][side * NFF * NF + ab];
+      T uK[5], uN[5], qq[15], out[5];
 #pragma unroll
-    for (int k = 0; k < 15; ++k) qq[k] = fb[(10 + k) * NF];
-    face_flux<T, D>(uK, uN, qq, side ? T(1) : T(-1), bg, out);
-    const T wt = so->w[ab % N] * so->w[ab / N] * T(m.ialpha[D]);
+      for (int s = 0; s < 5; ++s) { uK[s] = fb[s * NF]; uN[s] = fb[(5 + s) * NF]; }
 #pragma unroll
-    for (int s = 0; s < 5; ++s) fb[s * NF] = out[s] * wt;
+      for (int k = 0; k < 15; ++k) qq[k] = fb[(10 + k) * NF];
+      face_flux<T, D>(uK, uN, qq, side ? T(1) : T(-1), bg, out);
+      const T wt = so->w[ab % N] * so->w[ab / N] * T(m.ialpha[D]);
+#pragma unroll
+      for (int s = 0; s < 5; ++s) fb[s * NF] = out[s] * wt;
+    }
   }
   __syncthreads();
   // lift: I^T along a (in place)
   {
     Mat<N, T> I;
     I.load(so->I);
-    for (int w = tid; w < ne * 2 * NSTATE * N; w += NT) {
-      const int e = w / (2 * NSTATE * N);
-      const int r = w - e * (2 * NSTATE * N);
-      const int side = r / (NSTATE * N);
-      const int s = (r / N) % NSTATE;
-      const int qb = r % N;
-      T *p = &sbuf[e][(side * NFF + s) * NF + N * qb];
-      T v[N];
 #pragma unroll
-      for (int q = 0; q < N; ++q) v[q] = p[q];
+    for (int it = 0; it < FI::LL; ++it) {
+      const int li = fi.l(it);
+      if (li >= 0) {
+        const int e = li & 31, side = (li >> 5) & 1, s = (li >> 6) & 7, qb = li >> 9;
+        T *p = &sbuf[e][(side * NFF + s) * NF + N * qb];
+        T v[N];
 #pragma unroll
-      for (int i = 0; i < N; ++i) {
-        T acc = T(0);
+        for (int q = 0; q < N; ++q) v[q] = p[q];
 #pragma unroll
-        for (int q = 0; q < N; ++q) acc += I.v[q][i] * v[q];
-        p[i] = acc;
+        for (int i = 0; i < N; ++i) {
+          T acc = T(0);
+#pragma unroll
+          for (int q = 0; q < N; ++q) acc += I.v[q][i] * v[q];
+          p[i] = acc;
+        }
       }
     }
   }
@@ -615,23 +690,23 @@ __device__ inline void rhs_face_dir(const UT *__restrict__ U, const T *__restric
   {
     Mat<N, T> I;
     I.load(so->I);
-    for (int w = tid; w < ne * 2 * NSTATE * N; w += NT) {
-      const int e = w / (2 * NSTATE * N);
-      const int r = w - e * (2 * NSTATE * N);
-      const int side = r / (NSTATE * N);
-      const int s = (r / N) % NSTATE;
-      const int a = r % N;
-      const T *p = &sbuf[e][(side * NFF + s) * NF + a];
-      T v[N];
 #pragma unroll
-      for (int q = 0; q < N; ++q) v[q] = p[q * N];
-      const int off = side ? (N - 1) * st_d : 0;
+    for (int it = 0; it < FI::LL; ++it) {
+      const int li = fi.l(it);
+      if (li >= 0) {
+        const int e = li & 31, side = (li >> 5) & 1, s = (li >> 6) & 7, a = li >> 9;
+        const T *p = &sbuf[e][(side * NFF + s) * NF + a];
+        T v[N];
 #pragma unroll
-      for (int b = 0; b < N; ++b) {
-        T acc = T(0);
+        for (int q = 0; q < N; ++q) v[q] = p[q * N];
+        T *dst = &sr[e][s][a * ast + (side ? (N - 1) * st_d : 0)];
 #pragma unroll
-        for (int q = 0; q < N; ++q) acc += I.v[q][b] * v[q];
-        sr[e][s][line_base<N>(D, a, b) + off] -= acc;
+        for (int b = 0; b < N; ++b) {
+          T acc = T(0);
+#pragma unroll
+          for (int q = 0; q < N; ++q) acc += I.v[q][b] * v[q];
+          dst[b * bst] -= acc;
+        }
       }
     }
   }
@@ -649,12 +724,14 @@ ldg_rhs_kernel(const UT *__restrict__ U, const T *__restrict__ Q, LaunchArgs la,
   __shared__ double sred[NT];
   __shared__ SOps<N, T> sops;
   __shared__ long snb[E][6];
+  __shared__ const UT *pU[E][7];      // per element: own (0) and face neighbours' (1 + face) state
+  __shared__ const T *pQ[E][7];       // and gradient
   const int lb = logical_block(la);
   if (lb < 0) return;
   MeshArgs m;
   long kl;
   int ne, ec[3];
-  const InstDesc &d = block_instance(la, lb, m, ec, kl, ne);
+  const InstDesc &d = block_instance<E>(la, lb, m, ec, kl, ne);
   if (ne == 0) {                      // padding block of a partial brick: empty partial sums
     if (threadIdx.x == 0) {
       if (st.ke_part) st.ke_part[lb] = 0.0;
@@ -680,12 +757,20 @@ ldg_rhs_kernel(const UT *__restrict__ U, const T *__restrict__ Q, LaunchArgs la,
   fill_neighbors(m, ec, d.elem_off, ne, snb, tid);
   load_sops<N, T>(&sops, opp, tid);
   for (int w = tid; w < ne * NSTATE * NN; w += NT) (&sr[0][0][0])[w] = T(0);
+  const FaceItems<N, E> fi(ne, tid);
+  __syncthreads();
+  if (tid < ne * 7) {
+    const int e = tid / 7, f = tid % 7;
+    const long K = f ? snb[e][f - 1] : k0 + e;
+    pU[e][f] = U + K * (NSTATE * NN);
+    pQ[e][f] = Q + K * (NGRAD * NN);
+  }
   __syncthreads();
 
   // ================= faces, one direction at a time =================
-  rhs_face_dir<N, E, T, 0, UT>(U, Q, m, so, bg, sbuf, sr, snb, k0, ne, tid, brho, bE);
-  rhs_face_dir<N, E, T, 1, UT>(U, Q, m, so, bg, sbuf, sr, snb, k0, ne, tid, brho, bE);
-  rhs_face_dir<N, E, T, 2, UT>(U, Q, m, so, bg, sbuf, sr, snb, k0, ne, tid, brho, bE);
+  rhs_face_dir<N, E, T, 0, UT>(pU, pQ, fi, m, so, bg, sbuf, sr, brho, bE);
+  rhs_face_dir<N, E, T, 1, UT>(pU, pQ, fi, m, so, bg, sbuf, sr, brho, bE);
+  rhs_face_dir<N, E, T, 2, UT>(pU, pQ, fi, m, so, bg, sbuf, sr, brho, bE);
 
   // ================= volume =================
   // load U (minus the background) and Q x-lines from HBM, all loads of the
@@ -702,14 +787,13 @@ ldg_rhs_kernel(const UT *__restrict__ U, const T *__restrict__ Q, LaunchArgs la,
         const int r = w - e * (NV * NF);
         const int f = r / NF;
         const int yz = r % NF;
-        const long K = k0 + e;
         if (f < NSTATE) {
-          const UT *src = U + (K * NSTATE + f) * NN + N * yz;
+          const UT *src = pU[e][0] + (f * NN + N * yz);
           const double bgs = f == 0 ? brho : f == 4 ? bE : 0.0;
 #pragma unroll
           for (int i = 0; i < N; ++i) v[it][i] = T((double)src[i] - bgs);
         } else {
-          const T *src = Q + (K * NGRAD + (f - NSTATE)) * NN + N * yz;
+          const T *src = pQ[e][0] + ((f - NSTATE) * NN + N * yz);
 #pragma unroll
           for (int i = 0; i < N; ++i) v[it][i] = src[i];
         }
