@@ -713,6 +713,31 @@ __device__ inline void rhs_face_dir(const UT *const (*pU)[7], const T *const (*p
   __syncthreads();
 }
 
+// Work mapping of the volume-line loops (E elements x X = 20 fields x N^2
+// lines): per-element chunks of NT lines when one element fills at least
+// 3/4 of its chunks (p >= 4) -- the element of an item is then a
+// compile-time constant and the line decomposition, which depends only on
+// the thread and the chunk, is shared by all elements -- otherwise the
+// lines of all elements packed densely.  Items past the block have e >= ne
+// or r >= X.
+template <int N, int E>
+struct VolMap {
+  static constexpr int X = (NSTATE + NGRAD) * N * N;
+  static constexpr int C = (X + NT - 1) / NT;
+  static constexpr bool PER_E = 4 * X >= 3 * C * NT;
+  static constexpr int ITEMS = PER_E ? E * C : (E * X + NT - 1) / NT;
+  __device__ static inline void item(int it, int tid, int &e, int &r) {
+    if (PER_E) {
+      e = it / C;
+      r = tid + (it % C) * NT;
+    } else {
+      const int w = tid + it * NT;
+      e = w / X;
+      r = w - e * X;
+    }
+  }
+};
+
 template <int N, int E, typename T, typename UT>
 __global__ void __launch_bounds__(NT, (RhsLds<N, T>::WAVES))
 ldg_rhs_kernel(const UT *__restrict__ U, const T *__restrict__ Q, LaunchArgs la,
@@ -775,25 +800,23 @@ ldg_rhs_kernel(const UT *__restrict__ U, const T *__restrict__ Q, LaunchArgs la,
   // ================= volume =================
   // load U (minus the background) and Q x-lines from HBM, all loads of the
   // thread first, and interpolate along x
+  using VM = VolMap<N, E>;
   {
-    constexpr int VL = (E * NV * NF + NT - 1) / NT;
-    const int tot = ne * NV * NF;
-    T v[VL][N];
+    T v[VM::ITEMS][N];
 #pragma unroll
-    for (int it = 0; it < VL; ++it) {
-      const int w = tid + it * NT;
-      if (w < tot) {
-        const int e = w / (NV * NF);
-        const int r = w - e * (NV * NF);
+    for (int it = 0; it < VM::ITEMS; ++it) {
+      int e, r;
+      VM::item(it, tid, e, r);
+      if (e < ne && r < VM::X) {
         const int f = r / NF;
         const int yz = r % NF;
         if (f < NSTATE) {
-          const UT *src = pU[e][0] + (f * NN + N * yz);
+          const UT *src = (VM::PER_E ? U + (k0 + e) * (NSTATE * NN) : pU[e][0]) + (f * NN + N * yz);
           const double bgs = f == 0 ? brho : f == 4 ? bE : 0.0;
 #pragma unroll
           for (int i = 0; i < N; ++i) v[it][i] = T((double)src[i] - bgs);
         } else {
-          const T *src = pQ[e][0] + ((f - NSTATE) * NN + N * yz);
+          const T *src = (VM::PER_E ? Q + (k0 + e) * (NGRAD * NN) : pQ[e][0]) + ((f - NSTATE) * NN + N * yz);
 #pragma unroll
           for (int i = 0; i < N; ++i) v[it][i] = src[i];
         }
@@ -802,11 +825,10 @@ ldg_rhs_kernel(const UT *__restrict__ U, const T *__restrict__ Q, LaunchArgs la,
     Mat<N, T> I;
     I.load(so->I);
 #pragma unroll
-    for (int it = 0; it < VL; ++it) {
-      const int w = tid + it * NT;
-      if (w < tot) {
-        const int e = w / (NV * NF);
-        const int r = w - e * (NV * NF);
+    for (int it = 0; it < VM::ITEMS; ++it) {
+      int e, r;
+      VM::item(it, tid, e, r);
+      if (e < ne && r < VM::X) {
         T *dst = &sbuf[e][(r / NF) * NN + N * (r % NF)];
 #pragma unroll
         for (int q = 0; q < N; ++q) {
@@ -825,22 +847,25 @@ ldg_rhs_kernel(const UT *__restrict__ U, const T *__restrict__ Q, LaunchArgs la,
     {
       Mat<N, T> I;
       I.load(so->I);
-      for (int w = tid; w < ne * NV * NF; w += NT) {
-        const int e = w / (NV * NF);
-        const int r = w - e * (NV * NF);
-        const int f = r / NF;
-        const int ab = r % NF;
-        const int ls = line_stride<N>(dd);
-        T *p = &sbuf[e][f * NN + line_base<N>(dd, ab % N, ab / N)];
-        T v[N];
+      const int ls = line_stride<N>(dd);
 #pragma unroll
-        for (int i = 0; i < N; ++i) v[i] = p[i * ls];
+      for (int it = 0; it < VM::ITEMS; ++it) {
+        int e, r;
+        VM::item(it, tid, e, r);
+        if (e < ne && r < VM::X) {
+          const int f = r / NF;
+          const int ab = r % NF;
+          T *p = &sbuf[e][f * NN + line_base<N>(dd, ab % N, ab / N)];
+          T v[N];
 #pragma unroll
-        for (int q = 0; q < N; ++q) {
-          T acc = T(0);
+          for (int i = 0; i < N; ++i) v[i] = p[i * ls];
 #pragma unroll
-          for (int i = 0; i < N; ++i) acc += I.v[q][i] * v[i];
-          p[q * ls] = acc;
+          for (int q = 0; q < N; ++q) {
+            T acc = T(0);
+#pragma unroll
+            for (int i = 0; i < N; ++i) acc += I.v[q][i] * v[i];
+            p[q * ls] = acc;
+          }
         }
       }
     }
