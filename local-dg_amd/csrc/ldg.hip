@@ -60,8 +60,20 @@ int cand_e(int n, bool rhs, int c) {
   return 1;
 }
 
+// largest change made by symmetrize (A is replaced by its symmetric part)
 template <int N>
-Ops<N, double> make_ops(const Basis1D &B) {
+double symmetrize_checked(double (&A)[N][N], int S) {
+  double B[N][N], dev = 0.0;
+  for (int a = 0; a < N; ++a)
+    for (int b = 0; b < N; ++b) B[a][b] = A[a][b];
+  symmetrize<N>(A, S);
+  for (int a = 0; a < N; ++a)
+    for (int b = 0; b < N; ++b) dev = std::max(dev, std::fabs(A[a][b] - B[a][b]));
+  return dev;
+}
+
+template <int N>
+Ops<N, double> make_ops(const Basis1D &B, double &asym) {
   Ops<N, double> o;
   for (int a = 0; a < N; ++a) {
     for (int b = 0; b < N; ++b) {
@@ -73,6 +85,22 @@ Ops<N, double> make_ops(const Basis1D &B) {
     o.L0[a] = B.at(B.M1inv, a, 0);
     o.w[a] = B.w[a];
   }
+  // sweep operators in even/odd split form, from their (anti-)centro-symmetric
+  // parts (the node sets are symmetric; this removes construction round-off)
+  double I[N][N], It[N][N], Dt[N][N], Mi[N][N];
+  for (int a = 0; a < N; ++a)
+    for (int b = 0; b < N; ++b) {
+      I[a][b] = o.I[a][b];
+      It[a][b] = o.I[b][a];
+      Dt[a][b] = o.D[b][a];
+      Mi[a][b] = o.Minv[a][b];
+    }
+  asym = std::max(std::max(symmetrize_checked<N>(I, 1), symmetrize_checked<N>(It, 1)),
+                  std::max(symmetrize_checked<N>(Dt, -1), symmetrize_checked<N>(Mi, 1)));
+  make_symmat<N, double>(I, o.sI);
+  make_symmat<N, double>(It, o.sIt);
+  make_symmat<N, double>(Dt, o.sDt);
+  make_symmat<N, double>(Mi, o.sMi);
   return o;
 }
 
@@ -181,7 +209,9 @@ int launch_rhs(ldg_handle *h, Group &g, StageIn U, const StageArgs &st, int c) {
 
 template <int N>
 int upload_ops_t(ldg_handle *h, Group &g) {
-  const Ops<N, double> o = make_ops<N>(g.basis);
+  double asym = 0.0;
+  const Ops<N, double> o = make_ops<N>(g.basis, asym);
+  if (!(asym < 1e-9)) return fail(h, LDG_EINVAL, "1D operators are not centro-symmetric (node sets must be symmetric)");
   HIPCHECK(h, hipMalloc(&g.ops, sizeof(o)));
   HIPCHECK(h, hipMemcpy(g.ops, &o, sizeof(o), hipMemcpyHostToDevice));
   return LDG_OK;

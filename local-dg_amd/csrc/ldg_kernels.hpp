@@ -28,6 +28,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include "symop.hpp"
+
 namespace ldg {
 
 constexpr int NSTATE = 5;
@@ -43,6 +45,8 @@ struct Ops {
   T Minv[N][N];   // inverse 1D mass matrix
   T L0[N];        // Minv[:, 0]: lift of the jump at the left face
   T w[N];         // Gauss weights
+  // the sweep operators in even/odd split form: I, I^T, D^T, Minv
+  SymMat<N, T> sI, sIt, sDt, sMi;
 };
 
 struct MeshArgs {
@@ -472,19 +476,22 @@ struct RhsLds {
 // SGPRs nor spill).
 template <int N, typename T>
 struct SOps {
-  T I[N][N], D[N][N], w[N];
-  double Minv[N][N], wd[N];
+  SymMat<N, T> I, It, Dt;   // interpolation, its transpose, D^T (compute type)
+  SymMat<N, double> Mi;     // inverse mass (FP64)
+  T w[N];                   // Gauss weights (compute type and FP64)
+  double wd[N];
 };
 
 template <int N, typename T>
 __device__ inline void load_sops(SOps<N, T> *so, const Ops<N, double> *op, int tid) {
-  for (int t = tid; t < N * N; t += NT) {
-    const int a = t / N, b = t % N;
-    so->I[a][b] = T(op->I[a][b]);
-    so->D[a][b] = T(op->D[a][b]);
-    so->Minv[a][b] = op->Minv[a][b];
+  if (tid == 0) convert_symmat(so->I, op->sI);
+  if (tid == 1) convert_symmat(so->It, op->sIt);
+  if (tid == 2) convert_symmat(so->Dt, op->sDt);
+  if (tid == 3) so->Mi = op->sMi;
+  if (tid >= 64 && tid < 64 + N) {
+    so->w[tid - 64] = T(op->w[tid - 64]);
+    so->wd[tid - 64] = op->w[tid - 64];
   }
-  if (tid < N) { so->w[tid] = T(op->w[tid]); so->wd[tid] = op->w[tid]; }
 }
 
 template <int N, typename T>
@@ -598,7 +605,7 @@ __device__ inline void rhs_face_dir(const UT *const (*pU)[7], const T *const (*p
         }
       }
     }
-    Mat<N, T> I;
+    SymReg<N, T> I;
     I.load(so->I);
 #pragma unroll
     for (int it = 0; it < FI::GL; ++it) {
@@ -607,20 +614,17 @@ __device__ inline void rhs_face_dir(const UT *const (*pU)[7], const T *const (*p
         const int e = gi & 31, side = (gi >> 5) & 1, k = (gi >> 7) & 31, b = gi >> 12;
         const int slot = side * NFF + ((gi & 64) ? FI::NU + k : k);
         T *dst = &sbuf[e][slot * NF + N * b];
+        T o[N];
+        sym_apply<N, 1, T>(I.m, v[it], o);
 #pragma unroll
-        for (int q = 0; q < N; ++q) {
-          T acc = T(0);
-#pragma unroll
-          for (int a = 0; a < N; ++a) acc += I.v[q][a] * v[it][a];
-          dst[q] = acc;
-        }
+        for (int q = 0; q < N; ++q) dst[q] = o[q];
       }
     }
   }
   __syncthreads();
   // interpolate along b (in place)
   {
-    Mat<N, T> I;
+    SymReg<N, T> I;
     I.load(so->I);
 #pragma unroll
     for (int it = 0; it < FI::GL; ++it) {
@@ -629,16 +633,12 @@ __device__ inline void rhs_face_dir(const UT *const (*pU)[7], const T *const (*p
         const int e = gi & 31, side = (gi >> 5) & 1, k = (gi >> 7) & 31, qa = gi >> 12;
         const int slot = side * NFF + ((gi & 64) ? FI::NU + k : k);
         T *p = &sbuf[e][slot * NF + qa];
-        T v[N];
+        T v[N], o[N];
 #pragma unroll
         for (int b = 0; b < N; ++b) v[b] = p[b * N];
+        sym_apply<N, 1, T>(I.m, v, o);
 #pragma unroll
-        for (int q = 0; q < N; ++q) {
-          T acc = T(0);
-#pragma unroll
-          for (int b = 0; b < N; ++b) acc += I.v[q][b] * v[b];
-          p[q * N] = acc;
-        }
+        for (int q = 0; q < N; ++q) p[q * N] = o[q];
       }
     }
   }
@@ -664,49 +664,41 @@ __device__ inline void rhs_face_dir(const UT *const (*pU)[7], const T *const (*p
   __syncthreads();
   // lift: I^T along a (in place)
   {
-    Mat<N, T> I;
-    I.load(so->I);
+    SymReg<N, T> It;
+    It.load(so->It);
 #pragma unroll
     for (int it = 0; it < FI::LL; ++it) {
       const int li = fi.l(it);
       if (li >= 0) {
         const int e = li & 31, side = (li >> 5) & 1, s = (li >> 6) & 7, qb = li >> 9;
         T *p = &sbuf[e][(side * NFF + s) * NF + N * qb];
-        T v[N];
+        T v[N], o[N];
 #pragma unroll
         for (int q = 0; q < N; ++q) v[q] = p[q];
+        sym_apply<N, 1, T>(It.m, v, o);
 #pragma unroll
-        for (int i = 0; i < N; ++i) {
-          T acc = T(0);
-#pragma unroll
-          for (int q = 0; q < N; ++q) acc += I.v[q][i] * v[q];
-          p[i] = acc;
-        }
+        for (int i = 0; i < N; ++i) p[i] = o[i];
       }
     }
   }
   __syncthreads();
   // I^T along b, subtracted into the face nodes of the residual
   {
-    Mat<N, T> I;
-    I.load(so->I);
+    SymReg<N, T> It;
+    It.load(so->It);
 #pragma unroll
     for (int it = 0; it < FI::LL; ++it) {
       const int li = fi.l(it);
       if (li >= 0) {
         const int e = li & 31, side = (li >> 5) & 1, s = (li >> 6) & 7, a = li >> 9;
         const T *p = &sbuf[e][(side * NFF + s) * NF + a];
-        T v[N];
+        T v[N], o[N];
 #pragma unroll
         for (int q = 0; q < N; ++q) v[q] = p[q * N];
+        sym_apply<N, 1, T>(It.m, v, o);
         T *dst = &sr[e][s][a * ast + (side ? (N - 1) * st_d : 0)];
 #pragma unroll
-        for (int b = 0; b < N; ++b) {
-          T acc = T(0);
-#pragma unroll
-          for (int q = 0; q < N; ++q) acc += I.v[q][b] * v[q];
-          dst[b * bst] -= acc;
-        }
+        for (int b = 0; b < N; ++b) dst[b * bst] -= o[b];
       }
     }
   }
@@ -743,7 +735,7 @@ __global__ void __launch_bounds__(NT, (RhsLds<N, T>::WAVES))
 ldg_rhs_kernel(const UT *__restrict__ U, const T *__restrict__ Q, LaunchArgs la,
                const Ops<N, double> *__restrict__ opp, StageArgs st) {
   using L = RhsLds<N, T>;
-  constexpr int NN = L::NN, NF = L::NF, NV = L::NV, BUF = L::BUF;
+  constexpr int NN = L::NN, NF = L::NF, BUF = L::BUF;
   __shared__ __align__(16) T sbuf[E][BUF];
   __shared__ T sr[E][NSTATE][NN];
   __shared__ double sred[NT];
@@ -822,7 +814,7 @@ ldg_rhs_kernel(const UT *__restrict__ U, const T *__restrict__ Q, LaunchArgs la,
         }
       }
     }
-    Mat<N, T> I;
+    SymReg<N, T> I;
     I.load(so->I);
 #pragma unroll
     for (int it = 0; it < VM::ITEMS; ++it) {
@@ -830,13 +822,10 @@ ldg_rhs_kernel(const UT *__restrict__ U, const T *__restrict__ Q, LaunchArgs la,
       VM::item(it, tid, e, r);
       if (e < ne && r < VM::X) {
         T *dst = &sbuf[e][(r / NF) * NN + N * (r % NF)];
+        T o[N];
+        sym_apply<N, 1, T>(I.m, v[it], o);
 #pragma unroll
-        for (int q = 0; q < N; ++q) {
-          T acc = T(0);
-#pragma unroll
-          for (int i = 0; i < N; ++i) acc += I.v[q][i] * v[it][i];
-          dst[q] = acc;
-        }
+        for (int q = 0; q < N; ++q) dst[q] = o[q];
       }
     }
   }
@@ -845,7 +834,7 @@ ldg_rhs_kernel(const UT *__restrict__ U, const T *__restrict__ Q, LaunchArgs la,
 #pragma unroll
   for (int dd = 1; dd < 3; ++dd) {
     {
-      Mat<N, T> I;
+      SymReg<N, T> I;
       I.load(so->I);
       const int ls = line_stride<N>(dd);
 #pragma unroll
@@ -856,16 +845,12 @@ ldg_rhs_kernel(const UT *__restrict__ U, const T *__restrict__ Q, LaunchArgs la,
           const int f = r / NF;
           const int ab = r % NF;
           T *p = &sbuf[e][f * NN + line_base<N>(dd, ab % N, ab / N)];
-          T v[N];
+          T v[N], o[N];
 #pragma unroll
           for (int i = 0; i < N; ++i) v[i] = p[i * ls];
+          sym_apply<N, 1, T>(I.m, v, o);
 #pragma unroll
-          for (int q = 0; q < N; ++q) {
-            T acc = T(0);
-#pragma unroll
-            for (int i = 0; i < N; ++i) acc += I.v[q][i] * v[i];
-            p[q * ls] = acc;
-          }
+          for (int q = 0; q < N; ++q) p[q * ls] = o[q];
         }
       }
     }
@@ -914,48 +899,40 @@ ldg_rhs_kernel(const UT *__restrict__ U, const T *__restrict__ Q, LaunchArgs la,
 
   // test-function contraction: z sweep (I^T on F_x, F_y; D^T on F_z)
   {
-    Mat<N, T> I;
-    I.load(so->I);
+    SymReg<N, T> It;
+    It.load(so->It);
     for (int w = tid; w < ne * 10 * NF; w += NT) {
       const int e = w / (10 * NF);
       const int r = w - e * (10 * NF);
       T *p = &sbuf[e][NSTATE * NN + (r / NF) * NN + r % NF];
-      T v[N];
+      T v[N], o[N];
 #pragma unroll
       for (int q = 0; q < N; ++q) v[q] = p[q * NF];
+      sym_apply<N, 1, T>(It.m, v, o);
 #pragma unroll
-      for (int i = 0; i < N; ++i) {
-        T acc = T(0);
-#pragma unroll
-        for (int q = 0; q < N; ++q) acc += I.v[q][i] * v[q];
-        p[i * NF] = acc;
-      }
+      for (int i = 0; i < N; ++i) p[i * NF] = o[i];
     }
   }
   {
-    Mat<N, T> Dm;
-    Dm.load(so->D);
+    SymReg<N, T> Dt;
+    Dt.load(so->Dt);
     for (int w = tid; w < ne * NSTATE * NF; w += NT) {
       const int e = w / (NSTATE * NF);
       const int r = w - e * (NSTATE * NF);
       T *p = &sbuf[e][(NSTATE + 10) * NN + (r / NF) * NN + r % NF];
-      T v[N];
+      T v[N], o[N];
 #pragma unroll
       for (int q = 0; q < N; ++q) v[q] = p[q * NF];
+      sym_apply<N, -1, T>(Dt.m, v, o);
 #pragma unroll
-      for (int i = 0; i < N; ++i) {
-        T acc = T(0);
-#pragma unroll
-        for (int q = 0; q < N; ++q) acc += Dm.v[q][i] * v[q];
-        p[i * NF] = acc;
-      }
+      for (int i = 0; i < N; ++i) p[i * NF] = o[i];
     }
   }
   __syncthreads();
   // y sweep: F_x <- I^T F_x ; G <- D^T F_y + I^T F_z (stored in the F_y slot)
   {
-    Mat<N, T> I;
-    I.load(so->I);
+    SymReg<N, T> It;
+    It.load(so->It);
     for (int w = tid; w < ne * NSTATE * NF; w += NT) {
       const int e = w / (NSTATE * NF);
       const int r = w - e * (NSTATE * NF);
@@ -965,19 +942,15 @@ ldg_rhs_kernel(const UT *__restrict__ U, const T *__restrict__ Q, LaunchArgs la,
       T *px = &sbuf[e][(NSTATE + s) * NN + base];
       T *py = &sbuf[e][(NSTATE + 5 + s) * NN + base];
       const T *pz = &sbuf[e][(NSTATE + 10 + s) * NN + base];
-      T vx[N], vy[N], vz[N];
+      T vx[N], vy[N], vz[N], ox[N], og[N];
 #pragma unroll
       for (int q = 0; q < N; ++q) { vx[q] = px[q * N]; vy[q] = py[q * N]; vz[q] = pz[q * N]; }
+      sym_apply<N, 1, T>(It.m, vx, ox);
+      sym_apply2<N, -1, 1, T>(so->Dt, vy, It.m, vz, og);
 #pragma unroll
       for (int i = 0; i < N; ++i) {
-        T ax = T(0), ag = T(0);
-#pragma unroll
-        for (int q = 0; q < N; ++q) {
-          ax += I.v[q][i] * vx[q];
-          ag += so->D[q][i] * vy[q] + I.v[q][i] * vz[q];
-        }
-        px[i * N] = ax;
-        py[i * N] = ag;
+        px[i * N] = ox[i];
+        py[i * N] = og[i];
       }
     }
   }
@@ -991,8 +964,8 @@ ldg_rhs_kernel(const UT *__restrict__ U, const T *__restrict__ Q, LaunchArgs la,
     constexpr int XI = (E * NSTATE * NF + NT - 1) / NT;
     const int tot = ne * NSTATE * NF;
     double res[XI][N];
-    Mat<N, T> I;
-    I.load(so->I);
+    SymReg<N, T> It;
+    It.load(so->It);
 #pragma unroll
     for (int it = 0; it < XI; ++it) {
       const int w = tid + it * NT;
@@ -1003,24 +976,14 @@ ldg_rhs_kernel(const UT *__restrict__ U, const T *__restrict__ Q, LaunchArgs la,
         const int base = N * (r % NF);
         const T *px = &sbuf[e][(NSTATE + s) * NN + base];
         const T *pg = &sbuf[e][(NSTATE + 5 + s) * NN + base];
-        T vx[N], vg[N];
+        T vx[N], vg[N], o[N];
         double rr[N];
 #pragma unroll
         for (int q = 0; q < N; ++q) { vx[q] = px[q]; vg[q] = pg[q]; }
+        sym_apply2<N, -1, 1, T>(so->Dt, vx, It.m, vg, o);
 #pragma unroll
-        for (int i = 0; i < N; ++i) {
-          T acc = sr[e][s][base + i];
-#pragma unroll
-          for (int q = 0; q < N; ++q) acc += so->D[q][i] * vx[q] + I.v[q][i] * vg[q];
-          rr[i] = (double)acc;
-        }
-#pragma unroll
-        for (int i = 0; i < N; ++i) {
-          double acc = 0.0;
-#pragma unroll
-          for (int j = 0; j < N; ++j) acc += so->Minv[i][j] * rr[j];
-          res[it][i] = acc;
-        }
+        for (int i = 0; i < N; ++i) rr[i] = (double)(sr[e][s][base + i] + o[i]);
+        sym_apply<N, 1, double>(so->Mi, rr, res[it]);
       }
     }
     __syncthreads();
@@ -1072,8 +1035,8 @@ ldg_rhs_kernel(const UT *__restrict__ U, const T *__restrict__ Q, LaunchArgs la,
 #pragma unroll
   for (int dd = 1; dd < 3; ++dd) {
     {
-      Mat<N, double> Mi;
-      Mi.load(so->Minv);
+      SymReg<N, double> Mi;
+      Mi.load(so->Mi);
       for (int w = tid; w < ne * NSTATE * NF; w += NT) {
         const int e = w / (NSTATE * NF);
         const int r = w - e * (NSTATE * NF);
@@ -1081,16 +1044,12 @@ ldg_rhs_kernel(const UT *__restrict__ U, const T *__restrict__ Q, LaunchArgs la,
         const int ab = r % NF;
         const int ls = line_stride<N>(dd);
         double *p = md(e) + s * NN + line_base<N>(dd, ab % N, ab / N);
-        double v[N];
+        double v[N], o[N];
 #pragma unroll
         for (int i = 0; i < N; ++i) v[i] = p[i * ls];
+        sym_apply<N, 1, double>(Mi.m, v, o);
 #pragma unroll
-        for (int i = 0; i < N; ++i) {
-          double acc = 0.0;
-#pragma unroll
-          for (int j = 0; j < N; ++j) acc += Mi.v[i][j] * v[j];
-          p[i * ls] = acc;
-        }
+        for (int i = 0; i < N; ++i) p[i * ls] = o[i];
       }
     }
     __syncthreads();
