@@ -510,43 +510,53 @@ struct Mat {
 // decomposed once per workgroup (constant divisions cost several integer
 // instructions each) and kept packed in registers; with many small items
 // per thread (p <= 3, many elements per workgroup) they are decomposed where
-// used, which needs fewer registers.  Packing:
-//   g  gather / interpolation lines: e | side << 5 | isQ << 6 | k << 7 | b << 12,
+// used, which needs fewer registers.  Each item carries the offset of its
+// line (or point) in the workgroup's face buffer (e BUF + ...), so the LDS
+// phases need no index arithmetic.  Packing (-1: an item past the block):
+//   g  gather / interpolation lines: lds | b << 13 | side << 16 | isQ << 17 | k << 18 | e << 22,
+//      lds = e BUF + slot NF + N b with slot = side NFF + (isQ ? 10 + k : k),
 //      k = trace field (0..4 own U, 5..9 neighbour U) or gradient field
 //      (0..14); the U lines of all elements come first, so that most waves
 //      take one branch of the gather
-//   l  lift lines: e | side << 5 | s << 6 | c << 9
-//   f  face points: e | side << 5 | ab << 6
-// and -1 for an item past the block's elements.
-template <int N, int E>
+//   l  lift lines: lds | c << 13 | side << 16 | s << 17 | e << 20,
+//      lds = e BUF + (side NFF + s) NF + N c
+//   f  face points: lds | a << 13 | b << 16 | side << 19 | e << 20,  lds = e BUF + side NFF NF + a + N b
+template <int N, int E, int BUF>
 struct FaceItems {
-  static constexpr int NU = 2 * NSTATE;                     // trace fields
+  static constexpr int NU = 2 * NSTATE, NFF = NU + NGRAD;   // trace fields, face fields
   static constexpr int PU = 2 * NU * N, PQ = 2 * NGRAD * N;  // lines per element
   static constexpr int GL = (E * (PU + PQ) + NT - 1) / NT;
   static constexpr int LL = (E * 2 * NSTATE * N + NT - 1) / NT;
   static constexpr int FL = (E * 2 * N * N + NT - 1) / NT;
   static constexpr bool KEEP = GL <= 2;
+  static_assert(E * BUF <= 8192 && E <= 16 && N <= 7, "face item packing");
   int ne, tid;
   int gk[KEEP ? GL : 1], lk[KEEP ? LL : 1], fk[KEEP ? FL : 1];
 
+  __device__ static inline int gpack(int e, int side, int isq, int k, int b) {
+    const int slot = side * NFF + (isq ? NU + k : k);
+    return (e * BUF + slot * N * N + N * b) | b << 13 | side << 16 | isq << 17 | k << 18 | e << 22;
+  }
   __device__ static inline int gcode(int w, int ne) {
     if (w < E * PU) {
       const int e = w / PU, r = w - e * PU;
-      return e < ne ? e | (r / (NU * N)) << 5 | ((r / N) % NU) << 7 | (r % N) << 12 : -1;
+      return e < ne ? gpack(e, r / (NU * N), 0, (r / N) % NU, r % N) : -1;
     }
     if (w < E * (PU + PQ)) {
       const int w2 = w - E * PU, e = w2 / PQ, r = w2 - e * PQ;
-      return e < ne ? e | (r / (NGRAD * N)) << 5 | 1 << 6 | ((r / N) % NGRAD) << 7 | (r % N) << 12 : -1;
+      return e < ne ? gpack(e, r / (NGRAD * N), 1, (r / N) % NGRAD, r % N) : -1;
     }
     return -1;
   }
   __device__ static inline int lcode(int w, int ne) {
     const int e = w / (2 * NSTATE * N), r = w - e * (2 * NSTATE * N);
-    return e < ne ? e | (r / (NSTATE * N)) << 5 | ((r / N) % NSTATE) << 6 | (r % N) << 9 : -1;
+    const int side = r / (NSTATE * N), s = (r / N) % NSTATE, c = r % N;
+    return e < ne ? (e * BUF + (side * NFF + s) * N * N + N * c) | c << 13 | side << 16 | s << 17 | e << 20 : -1;
   }
   __device__ static inline int fcode(int w, int ne) {
     const int e = w / (2 * N * N), r = w - e * (2 * N * N);
-    return e < ne ? e | (r / (N * N)) << 5 | (r % (N * N)) << 6 : -1;
+    const int side = r / (N * N), ab = r % (N * N);
+    return e < ne ? (e * BUF + side * NFF * N * N + ab) | (ab % N) << 13 | (ab / N) << 16 | side << 19 | e << 20 : -1;
   }
   __device__ inline FaceItems(int ne_, int tid_) : ne(ne_), tid(tid_) {
     if (KEEP) {
@@ -570,11 +580,13 @@ struct FaceItems {
 // nodes and subtract them from the residual sr.  pU / pQ: per element, its
 // own (0) and its face neighbours' (1 + face) state and gradient.
 template <int N, int E, typename T, int D, typename UT>
-__device__ inline void rhs_face_dir(const UT *const (*pU)[7], const T *const (*pQ)[7], const FaceItems<N, E> &fi,
+__device__ inline void rhs_face_dir(const UT *const (*pU)[7], const T *const (*pQ)[7],
+                                    const FaceItems<N, E, RhsLds<N, T>::BUF> &fi,
                                     const MeshArgs &m, const SOps<N, T> *__restrict__ so, const Bg<T> &bg,
                                     T (*__restrict__ sbuf)[RhsLds<N, T>::BUF], T (*__restrict__ sr)[NSTATE][N * N * N],
                                     double brho, double bE) {
-  using FI = FaceItems<N, E>;
+  using FI = FaceItems<N, E, RhsLds<N, T>::BUF>;
+  T *const sb = &sbuf[0][0];
   constexpr int NN = N * N * N, NF = N * N, NFF = RhsLds<N, T>::NFF;
   constexpr int st_d = D == 0 ? 1 : D == 1 ? N : N * N;
   constexpr int ast = D == 0 ? N : 1;          // memory stride of the face coordinate a
@@ -587,8 +599,8 @@ __device__ inline void rhs_face_dir(const UT *const (*pU)[7], const T *const (*p
     for (int it = 0; it < FI::GL; ++it) {
       const int gi = fi.g(it);
       if (gi >= 0) {
-        const int e = gi & 31, side = (gi >> 5) & 1, k = (gi >> 7) & 31, b = gi >> 12;
-        if (!(gi & 64)) {
+        const int e = gi >> 22, side = (gi >> 16) & 1, k = (gi >> 18) & 15, b = (gi >> 13) & 7;
+        if (!(gi & (1 << 17))) {
           // own trace on this face, or the neighbour's on the opposite face
           const bool nbr = k >= NSTATE;
           const int s = nbr ? k - NSTATE : k;
@@ -611,9 +623,7 @@ __device__ inline void rhs_face_dir(const UT *const (*pU)[7], const T *const (*p
     for (int it = 0; it < FI::GL; ++it) {
       const int gi = fi.g(it);
       if (gi >= 0) {
-        const int e = gi & 31, side = (gi >> 5) & 1, k = (gi >> 7) & 31, b = gi >> 12;
-        const int slot = side * NFF + ((gi & 64) ? FI::NU + k : k);
-        T *dst = &sbuf[e][slot * NF + N * b];
+        T *dst = sb + (gi & 8191);
         T o[N];
         sym_apply<N, 1, T>(I.m, v[it], o);
 #pragma unroll
@@ -630,9 +640,8 @@ __device__ inline void rhs_face_dir(const UT *const (*pU)[7], const T *const (*p
     for (int it = 0; it < FI::GL; ++it) {
       const int gi = fi.g(it);
       if (gi >= 0) {
-        const int e = gi & 31, side = (gi >> 5) & 1, k = (gi >> 7) & 31, qa = gi >> 12;
-        const int slot = side * NFF + ((gi & 64) ? FI::NU + k : k);
-        T *p = &sbuf[e][slot * NF + qa];
+        const int qa = (gi >> 13) & 7;
+        T *p = sb + ((gi & 8191) - (N - 1) * qa);
         T v[N], o[N];
 #pragma unroll
         for (int b = 0; b < N; ++b) v[b] = p[b * N];
@@ -648,15 +657,15 @@ __device__ inline void rhs_face_dir(const UT *const (*pU)[7], const T *const (*p
   for (int it = 0; it < FI::FL; ++it) {
     const int fp = fi.f(it);
     if (fp >= 0) {
-      const int e = fp & 31, side = (fp >> 5) & 1, ab = fp >> 6;
-      T *fb = &sbuf[e][side * NFF * NF + ab];
+      const int side = (fp >> 19) & 1;
+      T *fb = sb + (fp & 8191);
       T uK[5], uN[5], qq[15], out[5];
 #pragma unroll
       for (int s = 0; s < 5; ++s) { uK[s] = fb[s * NF]; uN[s] = fb[(5 + s) * NF]; }
 #pragma unroll
       for (int k = 0; k < 15; ++k) qq[k] = fb[(10 + k) * NF];
       face_flux<T, D>(uK, uN, qq, side ? T(1) : T(-1), bg, out);
-      const T wt = so->w[ab % N] * so->w[ab / N] * T(m.ialpha[D]);
+      const T wt = so->w[(fp >> 13) & 7] * so->w[(fp >> 16) & 7] * T(m.ialpha[D]);
 #pragma unroll
       for (int s = 0; s < 5; ++s) fb[s * NF] = out[s] * wt;
     }
@@ -670,8 +679,7 @@ __device__ inline void rhs_face_dir(const UT *const (*pU)[7], const T *const (*p
     for (int it = 0; it < FI::LL; ++it) {
       const int li = fi.l(it);
       if (li >= 0) {
-        const int e = li & 31, side = (li >> 5) & 1, s = (li >> 6) & 7, qb = li >> 9;
-        T *p = &sbuf[e][(side * NFF + s) * NF + N * qb];
+        T *p = sb + (li & 8191);
         T v[N], o[N];
 #pragma unroll
         for (int q = 0; q < N; ++q) v[q] = p[q];
@@ -690,8 +698,8 @@ __device__ inline void rhs_face_dir(const UT *const (*pU)[7], const T *const (*p
     for (int it = 0; it < FI::LL; ++it) {
       const int li = fi.l(it);
       if (li >= 0) {
-        const int e = li & 31, side = (li >> 5) & 1, s = (li >> 6) & 7, a = li >> 9;
-        const T *p = &sbuf[e][(side * NFF + s) * NF + a];
+        const int a = (li >> 13) & 7, side = (li >> 16) & 1, s = (li >> 17) & 7, e = li >> 20;
+        const T *p = sb + ((li & 8191) - (N - 1) * a);
         T v[N], o[N];
 #pragma unroll
         for (int q = 0; q < N; ++q) v[q] = p[q * N];
@@ -774,7 +782,7 @@ ldg_rhs_kernel(const UT *__restrict__ U, const T *__restrict__ Q, LaunchArgs la,
   fill_neighbors(m, ec, d.elem_off, ne, snb, tid);
   load_sops<N, T>(&sops, opp, tid);
   for (int w = tid; w < ne * NSTATE * NN; w += NT) (&sr[0][0][0])[w] = T(0);
-  const FaceItems<N, E> fi(ne, tid);
+  const FaceItems<N, E, BUF> fi(ne, tid);
   __syncthreads();
   if (tid < ne * 7) {
     const int e = tid / 7, f = tid % 7;
@@ -793,6 +801,26 @@ ldg_rhs_kernel(const UT *__restrict__ U, const T *__restrict__ Q, LaunchArgs la,
   // load U (minus the background) and Q x-lines from HBM, all loads of the
   // thread first, and interpolate along x
   using VM = VolMap<N, E>;
+  // start of a volume line in its element's field block (field f, line ab):
+  // f NN + line_base(dd, a, b).  With the per-element mapping the thread's
+  // lines are the same in every element: decomposed once, for x, y and z.
+  int vl[3][VM::PER_E ? VM::C : 1];
+  if constexpr (VM::PER_E) {
+#pragma unroll
+    for (int c = 0; c < VM::C; ++c) {
+      const int r = tid + c * NT, f = r / NF, ab = r % NF;
+#pragma unroll
+      for (int dd = 0; dd < 3; ++dd) vl[dd][c] = f * NN + line_base<N>(dd, ab % N, ab / N);
+    }
+  }
+  auto line_start = [&](int it, int dd, int r) -> int {
+    if constexpr (VM::PER_E) {
+      return vl[dd][it % VM::C];
+    } else {
+      const int ab = r % NF;
+      return (r / NF) * NN + line_base<N>(dd, ab % N, ab / N);
+    }
+  };
   {
     T v[VM::ITEMS][N];
 #pragma unroll
@@ -800,15 +828,14 @@ ldg_rhs_kernel(const UT *__restrict__ U, const T *__restrict__ Q, LaunchArgs la,
       int e, r;
       VM::item(it, tid, e, r);
       if (e < ne && r < VM::X) {
-        const int f = r / NF;
-        const int yz = r % NF;
-        if (f < NSTATE) {
-          const UT *src = (VM::PER_E ? U + (k0 + e) * (NSTATE * NN) : pU[e][0]) + (f * NN + N * yz);
-          const double bgs = f == 0 ? brho : f == 4 ? bE : 0.0;
+        const int o = line_start(it, 0, r);   // f NN + N (y, z line index): also the line's offset in HBM
+        if (o < NSTATE * NN) {
+          const UT *src = (VM::PER_E ? U + (k0 + e) * (NSTATE * NN) : pU[e][0]) + o;
+          const double bgs = o < NN ? brho : o >= 4 * NN ? bE : 0.0;
 #pragma unroll
           for (int i = 0; i < N; ++i) v[it][i] = T((double)src[i] - bgs);
         } else {
-          const T *src = (VM::PER_E ? Q + (k0 + e) * (NGRAD * NN) : pQ[e][0]) + ((f - NSTATE) * NN + N * yz);
+          const T *src = (VM::PER_E ? Q + (k0 + e) * (NGRAD * NN) : pQ[e][0]) + (o - NSTATE * NN);
 #pragma unroll
           for (int i = 0; i < N; ++i) v[it][i] = src[i];
         }
@@ -821,7 +848,7 @@ ldg_rhs_kernel(const UT *__restrict__ U, const T *__restrict__ Q, LaunchArgs la,
       int e, r;
       VM::item(it, tid, e, r);
       if (e < ne && r < VM::X) {
-        T *dst = &sbuf[e][(r / NF) * NN + N * (r % NF)];
+        T *dst = &sbuf[e][line_start(it, 0, r)];
         T o[N];
         sym_apply<N, 1, T>(I.m, v[it], o);
 #pragma unroll
@@ -842,9 +869,7 @@ ldg_rhs_kernel(const UT *__restrict__ U, const T *__restrict__ Q, LaunchArgs la,
         int e, r;
         VM::item(it, tid, e, r);
         if (e < ne && r < VM::X) {
-          const int f = r / NF;
-          const int ab = r % NF;
-          T *p = &sbuf[e][f * NN + line_base<N>(dd, ab % N, ab / N)];
+          T *p = &sbuf[e][line_start(it, dd, r)];
           T v[N], o[N];
 #pragma unroll
           for (int i = 0; i < N; ++i) v[i] = p[i * ls];
