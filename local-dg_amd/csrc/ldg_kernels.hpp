@@ -319,17 +319,18 @@ __device__ inline void face_flux(const T wK[5], const T wN[5], const T *q, T sgn
 }
 
 // ---------------------------------------------------------------------
-// Block-wide deterministic sum of one double per thread -> every thread.
+// Block-wide deterministic sum of one double per thread -> every thread: a
+// fixed butterfly inside each wave (every lane ends with the same wave
+// total), then the NT/64 wave totals in a fixed order through sw[NT/64].
+// A caller that sums again with the same sw puts a barrier in between.
 // ---------------------------------------------------------------------
-__device__ inline double block_sum(double v, double *sred) {
-  sred[threadIdx.x] = v;
-  __syncthreads();
+__device__ inline double block_sum(double v, double *sw) {
 #pragma unroll
-  for (int s = NT / 2; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s) sred[threadIdx.x] += sred[threadIdx.x + s];
-    __syncthreads();
-  }
-  return sred[0];
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  if ((threadIdx.x & 63) == 0) sw[threadIdx.x >> 6] = v;
+  __syncthreads();
+  static_assert(NT == 256, "four wave totals");
+  return (sw[0] + sw[1]) + (sw[2] + sw[3]);
 }
 
 // ---------------------------------------------------------------------
@@ -746,7 +747,7 @@ ldg_rhs_kernel(const UT *__restrict__ U, const T *__restrict__ Q, LaunchArgs la,
   constexpr int NN = L::NN, NF = L::NF, BUF = L::BUF;
   __shared__ __align__(16) T sbuf[E][BUF];
   __shared__ T sr[E][NSTATE][NN];
-  __shared__ double sred[NT];
+  __shared__ double sred[NT / 64];
   __shared__ SOps<N, T> sops;
   __shared__ long snb[E][6];
   __shared__ const UT *pU[E][7];      // per element: own (0) and face neighbours' (1 + face) state
@@ -1152,7 +1153,7 @@ static __global__ void __launch_bounds__(NT) ldg_finalize_kernel(const double *_
                                                             const InstDesc *__restrict__ inst, int row_stride,
                                                             const double *__restrict__ scale, double *__restrict__ out,
                                                             const int *__restrict__ step) {
-  __shared__ double sred[NT];
+  __shared__ double sred[NT / 64];
   const int i = blockIdx.x;
   const int b0 = inst[i].block_off, b1 = b0 + inst[i].nblocks;
   const long row = step ? *step : 0;
