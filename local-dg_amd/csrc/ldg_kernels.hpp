@@ -341,14 +341,15 @@ __device__ inline double block_sum(double v, double *sw) {
 // consecutive nodes, coalesced).
 // ---------------------------------------------------------------------
 // Occupancy target of the gradient kernel (waves per SIMD): 6 with an FP32
-// gradient (measured 0.79 vs 0.87 ms at 64^3, p = 4), 5 (the compiler's own
-// choice) with FP64; never more than its LDS footprint allows (one
-// workgroup = one wave per SIMD, 160 KiB of LDS per CU).
+// gradient up to p = 4 (measured 0.79 vs 0.87 ms at 64^3, p = 4), 5 (the
+// compiler's own choice) with FP64 and at p = 5 (where 6 spills); never
+// more than its LDS footprint allows (one workgroup = one wave per SIMD,
+// 160 KiB of LDS per CU).
 template <int N, int E, typename TQ>
 struct GradWaves {
   static constexpr int lds = 8 * (E * NSTATE * N * N * N + E * 3 * NSTATE * N * N + N * (N + 1) + 6 * E);
   static constexpr int by_lds = (160 * 1024) / lds;
-  static constexpr int want = sizeof(TQ) == 4 ? 6 : 5;
+  static constexpr int want = sizeof(TQ) == 4 && N <= 5 ? 6 : 5;
   static constexpr int value = want < by_lds ? want : by_lds;
 };
 
@@ -507,11 +508,12 @@ struct Mat {
 };
 
 // Per-thread work items of the face phases.  They are the same for the three
-// directions, so for the larger orders (a few items per thread) they are
-// decomposed once per workgroup (constant divisions cost several integer
-// instructions each) and kept packed in registers; with many small items
-// per thread (p <= 3, many elements per workgroup) they are decomposed where
-// used, which needs fewer registers.  Each item carries the offset of its
+// directions, so with up to four gather lines per thread (p = 3..5 at their
+// default tiling) they are decomposed once per workgroup (constant divisions
+// cost several integer instructions each) and kept packed in registers
+// (p = 3: rhs -5 %); with more lines per thread (p <= 2, many elements per
+// workgroup) and at p = 1 they are decomposed where used, since keeping
+// them costs more registers than it saves.  Each item carries the offset of its
 // line (or point) in the workgroup's face buffer (e BUF + ...), so the LDS
 // phases need no index arithmetic.  Packing (-1: an item past the block):
 //   g  gather / interpolation lines: lds | b << 13 | side << 16 | isQ << 17 | k << 18 | e << 22,
@@ -529,7 +531,7 @@ struct FaceItems {
   static constexpr int GL = (E * (PU + PQ) + NT - 1) / NT;
   static constexpr int LL = (E * 2 * NSTATE * N + NT - 1) / NT;
   static constexpr int FL = (E * 2 * N * N + NT - 1) / NT;
-  static constexpr bool KEEP = GL <= 2;
+  static constexpr bool KEEP = GL <= 4 && N >= 3;
   static_assert(E * BUF <= 8192 && E <= 16 && N <= 7, "face item packing");
   int ne, tid;
   int gk[KEEP ? GL : 1], lk[KEEP ? LL : 1], fk[KEEP ? FL : 1];

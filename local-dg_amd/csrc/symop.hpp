@@ -17,23 +17,25 @@
 // so only rows q < ceil(N/2) are evaluated: N = 5 takes 21 operations per
 // line instead of 25, N = 6 30 instead of 36, and a sum of two operators
 // (D^T a + I^T b, the fused contraction sweeps) 38 instead of 50 at N = 5.
-// This is the even-odd decomposition of spectral-element codes (Solomonoff,
-// "A fast algorithm for spectral differentiation", J. Comput. Phys. 98,
-// 1992; Kopriva, "Implementing Spectral Methods for Partial Differential
-// Equations", 2009).  For N <= 4 it saves nothing and the plain product is
-// used.
+// At N = 3, 4 the operation count is about the same, but the operator needs
+// half the coefficients (registers and LDS loads): measured rhs -15 % at
+// N = 4 and -4 % at N = 3 (mixed).  At N = 2 the split costs more and the
+// plain product is used.  This is the even-odd decomposition of
+// spectral-element codes (Solomonoff, "A fast algorithm for spectral
+// differentiation", J. Comput. Phys. 98, 1992; Kopriva, "Implementing
+// Spectral Methods for Partial Differential Equations", 2009).
 #pragma once
 
 #include <hip/hip_runtime.h>
 
 namespace ldg {
 
-// Rows q < Q = ceil(N/2) of A in split form, or A itself for the orders
-// where the plain product is as cheap (N <= 4).
+// Rows q < Q = ceil(N/2) of A in split form, or A itself for N = 2, where
+// the plain product is cheaper.
 template <int N, typename T>
 struct SymMat {
   static constexpr int H = N / 2, Q = (N + 1) / 2;
-  static constexpr bool SPLIT = N >= 5;
+  static constexpr bool SPLIT = N >= 3;
   static constexpr int QP = SPLIT ? Q : 1, HP = SPLIT ? H : 1, NA = SPLIT ? 1 : N;
   T P[QP][HP];   // split form (SPLIT)
   T R[QP][HP];
