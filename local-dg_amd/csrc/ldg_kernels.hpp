@@ -7,7 +7,8 @@
 //                    inverse mass and the fused RK4 stage epilogue
 //                    (solver.cpp:510-535 and 410-434)
 // Both stage a block of E consecutive elements in LDS, work in 1D "lines"
-// of N = p+1 values owned by one thread (in-place sum-factorised sweeps),
+// of N = p+1 values owned by one thread (in-place sum-factorised sweeps,
+// with the even/odd split of the centro-symmetric 1D operators, symop.hpp),
 // and read the periodic neighbours' face slices straight from HBM (the
 // on-device form of the single-rank halo exchange, solver.cpp:929-998).
 //
@@ -22,8 +23,10 @@
 // rounding.
 //
 // Data in HBM:
-//   U, u0, acc, unext : FP64 (node, state, element) -- the reference layout
-//   Q                 : TQ (node, s + 5*l, element)  -- 15 gradient fields per element
+//   u0, acc, unext : FP64 (node, state, element) -- the reference layout
+//   U (stage input): the same, or in the mixed path (RK stages 2-4) FP32
+//                    fluctuations about the background in that layout
+//   Q              : TQ (node, s + 5*l, element)  -- 15 gradient fields per element
 #pragma once
 
 #include <hip/hip_runtime.h>
