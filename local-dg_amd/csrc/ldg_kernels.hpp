@@ -375,56 +375,53 @@ __global__ void __launch_bounds__(NT, (GradWaves<N, E, TQ>::value)) ldg_grad_ker
   const int tid = threadIdx.x;
   fill_neighbors(m, ec, d.elem_off, ne, snb, tid);
 
+  // Q_l(i) = (Dn u)(i) + L0(i) (u_0 - u_nbr): the own trace u_0 is the line's
+  // first node, so its lift joins column 0 of Dn and only the neighbour's
+  // trace enters separately, with weight -L0(i) in column N
   if (tid < N * (N + 1)) {
     const int i = tid / (N + 1), j = tid % (N + 1);
-    sD[i][j] = j < N ? opp->Dn[i][j] : opp->L0[i];
+    sD[i][j] = j < N ? opp->Dn[i][j] + (j == 0 ? opp->L0[i] : 0.0) : -opp->L0[i];
   }
-  // the block's state (contiguous) into LDS: all of a thread's loads are
-  // issued before the first is stored.  The loads are unconditional (an
-  // index past the block is clamped to its last word) and converted only
-  // when stored: a load under a branch is waited for at the branch's end.
+  __syncthreads();   // neighbour table
+
+  // the block's state (contiguous) and the -d neighbours' traces on their
+  // positive faces into LDS, all of a thread's loads issued before the first
+  // is stored.  The loads are unconditional (an index past the block is
+  // clamped to its last word) and converted only when stored: a load under
+  // a branch is waited for at the branch's end.
   {
     constexpr int UL = (E * NSTATE * NN + NT - 1) / NT;
+    constexpr int JL = (E * 3 * NSTATE * NF + NT - 1) / NT;
     const UT *Ub = U + k0 * NSTATE * NN;
     const int tot = ne * NSTATE * NN;
-    UT v[UL];
+    // trace item w = ((e * 3 + d) * NSTATE + s) * NF + ab is also its index in sj
+    const int totj = ne * 3 * NSTATE * NF;
+    UT v[UL], vj[JL];
 #pragma unroll
     for (int it = 0; it < UL; ++it) {
       const int w = tid + it * NT;
       v[it] = Ub[w < tot ? w : tot - 1];
     }
 #pragma unroll
-    for (int it = 0; it < UL; ++it) {
-      const int w = tid + it * NT;
-      if (w < tot) (&su[0][0][0])[w] = (double)v[it];
-    }
-  }
-  __syncthreads();
-
-  // jump on the negative face of direction d: own trace minus the -d
-  // neighbour's trace on its positive face (neighbour loads batched too)
-  {
-    constexpr int JL = (E * 3 * NSTATE * NF + NT - 1) / NT;
-    // item w = ((e * 3 + d) * NSTATE + s) * NF + ab is also its index in sj
-    const int tot = ne * 3 * NSTATE * NF;
-    UT v[JL];
-    int own[JL];          // the own trace's index in su
-#pragma unroll
     for (int it = 0; it < JL; ++it) {
-      const int w = tid + it * NT < tot ? tid + it * NT : tot - 1;
+      const int w = tid + it * NT < totj ? tid + it * NT : totj - 1;
       const int e = w / (3 * NSTATE * NF);
       const int r = w - e * (3 * NSTATE * NF);
       const int d = r / (NSTATE * NF);
       const int s = (r / NF) % NSTATE;
       const int ab = r % NF;
       const int base = line_base<N>(d, ab % N, ab / N), st = line_stride<N>(d);
-      v[it] = U[snb[e][2 * d] * (NSTATE * NN) + (s * NN + base + (N - 1) * st)];
-      own[it] = (e * NSTATE + s) * NN + base;
+      vj[it] = U[snb[e][2 * d] * (NSTATE * NN) + (s * NN + base + (N - 1) * st)];
+    }
+#pragma unroll
+    for (int it = 0; it < UL; ++it) {
+      const int w = tid + it * NT;
+      if (w < tot) (&su[0][0][0])[w] = (double)v[it];
     }
 #pragma unroll
     for (int it = 0; it < JL; ++it) {
       const int w = tid + it * NT;
-      if (w < tot) (&sj[0][0][0][0])[w] = (&su[0][0][0])[own[it]] - (double)v[it];
+      if (w < totj) (&sj[0][0][0][0])[w] = (double)vj[it];
     }
   }
   __syncthreads();
