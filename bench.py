@@ -137,6 +137,57 @@ def ensemble_bench(ldg_amd, args):
     return out
 
 
+SAMPLE_ELEMENTS = 4096   # elements kept in a validation fingerprint (seeded choice)
+
+
+def validate_part(ldg_amd, p, nx, t_end, name, out):
+    """One path from the Taylor-Green state to t_end (reference dt rule,
+    rounded to end exactly at t_end); saves the KE history, the Gauss
+    diagnostics, per-variable L2 norms of the final state and the final
+    state on a seeded sample of elements, so that two runs too long for one
+    process can be compared afterwards (validate_compare)."""
+    prec = ldg_amd.LDG_FP64 if name == "fp64" else ldg_amd.LDG_MIXED
+    g = ldg_amd.LDG(p, nx, precision=prec)
+    g.initial_condition()
+    dt0 = g.time_step(0.01)
+    n = int(np.ceil(t_end / dt0))
+    dt = t_end / n
+    kes, done, t0 = [], 0, time.perf_counter()
+    while done < n:
+        c = min(500, n - done)
+        kes.append(g.step(dt, c, record_ke=True))
+        done += c
+        print(f"{name}: {done}/{n} steps", file=sys.stderr, flush=True)
+    wall = time.perf_counter() - t0
+    u = g.get_state()
+    d = g.diagnostics()
+    g.close()
+    # u is (K, 5, dofs): elements are the first axis
+    idx = np.sort(np.random.default_rng(20261203).choice(u.shape[0], size=min(SAMPLE_ELEMENTS, u.shape[0]), replace=False))
+    np.savez_compressed(out, ke=np.concatenate(kes), dt=dt, steps=n, t_end=t_end, wall=wall, idx=idx,
+                        sample=u[idx], norms=np.sqrt(np.sum(u ** 2, axis=(0, 2))),
+                        enstrophy=d["enstrophy"], ke_gauss=d["ke"], p=p, nx=nx)
+    print(json.dumps({"validate_part": name, "t_end": t_end, "steps": n, "dt": dt, "wall_s": wall,
+                      "ms_per_step": wall / n * 1e3, "out": out}), flush=True)
+
+
+def validate_compare(mixed_npz, fp64_npz):
+    """Errors of a mixed-precision fingerprint against an FP64 one of the
+    same run: relative L2 per variable on the sampled elements, the KE
+    history and the -dEk/dt series."""
+    a, b = np.load(mixed_npz), np.load(fp64_npz)
+    assert int(a["steps"]) == int(b["steps"]) and np.array_equal(a["idx"], b["idx"]), "different runs"
+    dt = float(b["dt"])
+    return {"t_end": float(b["t_end"]), "steps": int(b["steps"]), "dt": dt,
+            "sample_elements": int(len(b["idx"])),
+            "rel_l2_mixed_vs_fp64_sampled": rel_l2(a["sample"], b["sample"]),
+            "norm_rel_diff": (np.abs(a["norms"] - b["norms"]) / b["norms"]).tolist(),
+            "ke_rel_err": float(np.max(np.abs(a["ke"] - b["ke"]) / b["ke"])),
+            "dEk_dt_err": dissipation_err(a["ke"], b["ke"], dt),
+            "enstrophy_fp64": float(b["enstrophy"]), "enstrophy_mixed": float(a["enstrophy"]),
+            "wall_s": {"fp64": float(b["wall"]), "mixed": float(a["wall"])}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -151,6 +202,12 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--validate", type=float, default=0.0, metavar="T",
                     help="run both paths to time T and print their difference only")
+    ap.add_argument("--validate-part", choices=["fp64", "mixed"], default=None,
+                    help="with --validate T: run only this path to T and save its fingerprint to --out "
+                         "(KE history, enstrophy, norms, the state on a seeded sample of 4096 elements)")
+    ap.add_argument("--out", default=None, help="fingerprint file (.npz) for --validate-part")
+    ap.add_argument("--validate-compare", nargs=2, default=None, metavar=("MIXED_NPZ", "FP64_NPZ"),
+                    help="compare two --validate-part fingerprints and print the errors")
     ap.add_argument("--ensemble", type=int, default=0, metavar="N",
                     help="instead: an N-instance seeded ensemble (mixed precision, autotuned, graph vs eager)")
     ap.add_argument("--seed", type=int, default=0, help="ensemble sampler seed")
@@ -169,6 +226,10 @@ def main():
         print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8, "dispatches": 3}))
         return
 
+    if args.validate_compare:
+        print(json.dumps(validate_compare(*args.validate_compare)), flush=True)
+        return
+
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
               "the multi-GPU halo exchange is not built", file=sys.stderr)
@@ -179,6 +240,10 @@ def main():
 
     if args.ensemble:
         print(json.dumps(ensemble_bench(ldg_amd, args)), flush=True)
+        return
+
+    if args.validate > 0 and args.validate_part:
+        validate_part(ldg_amd, p, nx, args.validate, args.validate_part, args.out)
         return
 
     if args.validate > 0:

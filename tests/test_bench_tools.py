@@ -1,0 +1,54 @@
+"""Host-side helpers of bench.py: the algorithmic byte model and the split
+mixed-vs-FP64 validation (two runs saved as fingerprints, compared later)."""
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, ROOT)
+import bench  # noqa: E402
+
+
+def test_alg_bytes_per_dof():
+    # FP64: grad U 8 + Q 24; rhs 58 B stage average.  Mixed: FP32 stage
+    # inputs after stage 1 and an FP32 gradient.
+    assert bench.alg_bytes(1, False) == (32.0, 58.0)
+    assert bench.alg_bytes(1, True) == (17.0, 40.0)
+
+
+def _fingerprint(path, sample, ke, norms, dt=1e-3, steps=None, idx=None):
+    steps = len(ke) - 1 if steps is None else steps
+    idx = np.arange(sample.shape[0]) if idx is None else idx
+    np.savez_compressed(path, ke=ke, dt=dt, steps=steps, t_end=dt * steps, wall=1.0, idx=idx, sample=sample,
+                        norms=norms, enstrophy=0.5, ke_gauss=0.1, p=4, nx=8)
+
+
+def test_validate_compare_matches_direct_errors(tmp_path):
+    rng = np.random.default_rng(1)
+    ref = rng.standard_normal((16, 5, 125)) + 3.0     # (elements, variables, nodes)
+    mix = ref.copy()
+    mix[:, 1, :] += 1e-6 * rng.standard_normal((16, 125))   # only rho u differs
+    ke = 0.125 - 1e-3 * np.arange(11) ** 2 * 1e-2
+    ke_m = ke * (1 + 1e-7 * np.sin(np.arange(11)))
+    _fingerprint(tmp_path / "f.npz", ref, ke, np.ones(5))
+    _fingerprint(tmp_path / "m.npz", mix, ke_m, np.ones(5) * (1 + 1e-9))
+    out = bench.validate_compare(str(tmp_path / "m.npz"), str(tmp_path / "f.npz"))
+    err = out["rel_l2_mixed_vs_fp64_sampled"]
+    direct = np.linalg.norm(mix[:, 1, :] - ref[:, 1, :]) / np.linalg.norm(ref[:, 1, :])
+    assert err[0] == 0.0 and err[2] == 0.0 and err[3] == 0.0 and err[4] == 0.0
+    assert abs(err[1] - direct) <= 1e-12 * direct
+    assert abs(out["ke_rel_err"] - np.max(np.abs(ke_m - ke) / ke)) < 1e-18
+    assert out["steps"] == 10 and out["sample_elements"] == 16
+    assert max(out["norm_rel_diff"]) < 2e-9
+
+
+def test_validate_compare_refuses_different_runs(tmp_path):
+    a = np.ones((4, 5, 8))
+    _fingerprint(tmp_path / "a.npz", a, np.ones(5), np.ones(5))
+    _fingerprint(tmp_path / "b.npz", a, np.ones(7), np.ones(5))
+    try:
+        bench.validate_compare(str(tmp_path / "a.npz"), str(tmp_path / "b.npz"))
+    except AssertionError:
+        return
+    raise AssertionError("runs of different length were compared")
