@@ -590,7 +590,7 @@ __device__ inline void rhs_face_dir(const UT *const (*pU)[7], const T *const (*p
                                     double brho, double bE) {
   using FI = FaceItems<N, E, RhsLds<N, T>::BUF>;
   T *const sb = &sbuf[0][0];
-  constexpr int NN = N * N * N, NF = N * N, NFF = RhsLds<N, T>::NFF;
+  constexpr int NN = N * N * N, NF = N * N;
   constexpr int st_d = D == 0 ? 1 : D == 1 ? N : N * N;
   constexpr int ast = D == 0 ? N : 1;          // memory stride of the face coordinate a
   constexpr int bst = D == 2 ? N : N * N;      // memory stride of the face coordinate b

@@ -4,6 +4,9 @@
 //
 //   ldg_driver [-p 2] [-nx 10] [-L 1] [-tf 10] [-dtSnaps 30] [-cfl 0.01]
 //              [-Re 1600] [-M0 0.1] [-mixed] [-nosnap] [-out output] [-quiet]
+//              [-checkpoint FILE] [-resume FILE]
+// -checkpoint writes a binary checkpoint when the run ends or hits the wall
+// limit; -resume continues a run from one (same options otherwise).
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -15,6 +18,7 @@
 int main(int argc, char **argv) {
   int p = 2, nx = 10, dtSnaps = 30;
   double L = 1.0, tf = 10.0;
+  const char *resume = nullptr;
   ldg::SolverOptions opt;
   for (int i = 1; i < argc; ++i) {
     auto next = [&](const char *flag) -> const char * {
@@ -33,6 +37,8 @@ int main(int argc, char **argv) {
     else if (!std::strcmp(argv[i], "-nosnap")) opt.snapshots = false;
     else if (!std::strcmp(argv[i], "-out")) opt.outputDir = next("-out");
     else if (!std::strcmp(argv[i], "-quiet")) opt.verbose = false;
+    else if (!std::strcmp(argv[i], "-checkpoint")) opt.checkpointFile = next("-checkpoint");
+    else if (!std::strcmp(argv[i], "-resume")) resume = next("-resume");
     else { std::cerr << "unknown option " << argv[i] << std::endl; return 2; }
   }
   try {
@@ -40,6 +46,7 @@ int main(int argc, char **argv) {
     ldg::Mesh mesh(nx, nx, nx, ldg::Point(-size, -size, -size), ldg::Point(size, size, size));
     std::cout << "p = " << p << "\ntf = " << tf << "\nnx = " << nx << std::endl;
     ldg::Solver solver(p, dtSnaps, tf, L, mesh, opt);
+    if (resume) solver.loadCheckpoint(resume);
     solver.dgTimeStep();
   } catch (const std::exception &e) {
     std::cerr << "ERROR: " << e.what() << std::endl;

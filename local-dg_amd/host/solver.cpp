@@ -1,5 +1,6 @@
 #include "solver.hpp"
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <iostream>
@@ -13,7 +14,7 @@ namespace ldg {
 Solver::Solver() : Solver(2, 10, 1.0, 1.0, Mesh()) {}
 
 Solver::Solver(int p, int dtSnaps_, double tf_, double L, const Mesh &mesh_, const SolverOptions &opt_)
-    : mesh(mesh_), opt(opt_), tf(tf_), dtSnaps(dtSnaps_), order(p) {
+    : mesh(mesh_), opt(opt_), tf(tf_), L(L), dtSnaps(dtSnaps_), order(p) {
   ldg_config c;
   ldg_config_default(&c);
   c.p = p;
@@ -62,18 +63,46 @@ const darray &Solver::u() {
   return uHost;
 }
 
-void Solver::dgTimeStep() {
+void Solver::saveCheckpoint(const std::string &filename) {
+  Checkpoint c;
+  c.p = order; c.nx = mesh.nx; c.ny = mesh.ny; c.nz = mesh.nz; c.precision = opt.precision;
+  c.step = step; c.timesteps = timesteps;
+  c.dt = dt; c.tf = tf; c.L = L; c.Re = opt.Re; c.M0 = opt.M0; c.V0 = opt.V0; c.Pr = opt.Pr;
+  c.lo[0] = mesh.botLeft.x; c.lo[1] = mesh.botLeft.y; c.lo[2] = mesh.botLeft.z;
+  c.hi[0] = mesh.topRight.x; c.hi[1] = mesh.topRight.y; c.hi[2] = mesh.topRight.z;
+  const darray &uh = u();
+  c.u.assign(uh.data(), uh.data() + uh.size());
+  c.ke.assign(kes.begin(), kes.begin() + std::min<std::size_t>(kes.size(), (std::size_t)step));
+  if (!writeCheckpoint(filename, c)) throw std::runtime_error("could not write checkpoint " + filename);
+}
+
+void Solver::loadCheckpoint(const std::string &filename) {
+  Checkpoint c;
+  if (!readCheckpoint(filename, c)) throw std::runtime_error("could not read checkpoint " + filename);
+  const bool same = c.p == order && c.nx == mesh.nx && c.ny == mesh.ny && c.nz == mesh.nz && c.timesteps == timesteps &&
+                    c.dt == dt && c.Re == opt.Re && c.M0 == opt.M0 && c.V0 == opt.V0 && c.Pr == opt.Pr &&
+                    c.lo[0] == mesh.botLeft.x && c.hi[0] == mesh.topRight.x && c.lo[1] == mesh.botLeft.y &&
+                    c.hi[1] == mesh.topRight.y && c.lo[2] == mesh.botLeft.z && c.hi[2] == mesh.topRight.z;
+  if (!same) throw std::runtime_error("checkpoint " + filename + " is from a different run (order, mesh, physics or dt)");
+  check(ldg_set_state(h, c.u.data()), "ldg_set_state");
+  step = c.step;
   kes.assign(timesteps, 0.0);
+  std::copy(c.ke.begin(), c.ke.end(), kes.begin());
+}
+
+void Solver::dgTimeStep() {
+  if (step == 0) kes.assign(timesteps, 0.0);
   if (opt.verbose) std::cout << "Time stepping until tf = " << tf << std::endl;
   if (opt.snapshots) mkdir(opt.outputDir.c_str(), 0755);
   const auto start = std::chrono::steady_clock::now();
-  long iStep = 0;
+  long &iStep = step;
   while (iStep < timesteps) {
     const double elapsed = std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();
     if (elapsed > opt.wallLimitSeconds) {     // dump state and stop (solver.cpp:349-360)
       const std::string name = opt.outputDir + "/xyznavier.tmp";
       if (!initXYZVFile(name, 0, "u", 5) || !exportToXYZVFile(name, 0, coords, u()))
         throw std::runtime_error("could not write " + name);
+      if (!opt.checkpointFile.empty()) saveCheckpoint(opt.checkpointFile);
       kes.resize(iStep);
       break;
     }
@@ -88,6 +117,7 @@ void Solver::dgTimeStep() {
     check(ldg_step(h, dt, (int)chunk, kes.data() + iStep), "ldg_step");
     iStep += chunk;
   }
+  if (iStep == timesteps && !opt.checkpointFile.empty()) saveCheckpoint(opt.checkpointFile);
   // dissipation -(KE[i] - KE[i-2]) / (2 dt) (solver.cpp:437-440)
   keDiss.clear();
   for (std::size_t i = 2; i < kes.size(); ++i) keDiss.push_back(-(kes[i] - kes[i - 2]) / (2 * dt));

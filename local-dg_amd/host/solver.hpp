@@ -27,6 +27,7 @@ struct SolverOptions {
   std::string outputDir = "output";
   double wallLimitSeconds = 19.5 * 60 * 60;   // solver.cpp:351
   bool verbose = true;
+  std::string checkpointFile;     // if set: binary checkpoint at the end of dgTimeStep and at the wall limit
 };
 
 class Solver {
@@ -38,6 +39,13 @@ class Solver {
   Solver &operator=(const Solver &) = delete;
 
   void dgTimeStep();
+
+  // binary checkpoint / resume (io.hpp): the state, the step reached and
+  // the KE history so far; loadCheckpoint continues the same run (same
+  // order, mesh, physics and dt), and dgTimeStep then runs the rest of it
+  void saveCheckpoint(const std::string &filename);
+  void loadCheckpoint(const std::string &filename);
+  long currentStep() const { return step; }
 
   // results and state
   const std::vector<double> &kineticEnergy() const { return kes; }
@@ -53,8 +61,8 @@ class Solver {
   Mesh mesh;
   SolverOptions opt;
   ldg_handle *h = nullptr;
-  double tf, dt;
-  long timesteps;
+  double tf, dt, L;
+  long timesteps, step = 0;
   int dtSnaps, order, dofs_;
   darray uHost, coords;
   std::vector<double> kes, keDiss;

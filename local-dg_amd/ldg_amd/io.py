@@ -56,3 +56,61 @@ def node_coordinates(p, mesh):
     b = (ie + 1) * (hi - lo) / ns + lo
     scale = 0.5 * (r[inode] + 1.0)
     return a[:, None, :] + scale[None, :, :] * (b - a)[:, None, :]
+
+
+# ------------------------------------------------------------------ checkpoints
+# Binary checkpoint of a run, the layout of local-dg_amd/host/io.hpp
+# (writeCheckpoint / readCheckpoint), so that either side can resume the
+# other's runs.  No reference counterpart: the reference only dumps an XYZV
+# text file at its wall-clock limit (solver.cpp:349-360).
+CKPT_MAGIC = b"LDGCKPT1"
+_CKPT_HEAD = np.dtype([("magic", "S8"), ("ints", "<i4", 5), ("steps", "<i8", 2), ("reals", "<f8", 13)])
+
+
+def save_checkpoint(filename, u, *, p, nx, ny, nz, precision, step, timesteps, dt, tf, L, Re, M0, V0, Pr,
+                    lo, hi, ke):
+    """Write the state u ((K, 5, dofs), i.e. the reference's (dofs, 5, K)
+    column-major layout), the step reached and the KE history of steps
+    1..step."""
+    u = np.ascontiguousarray(u, dtype="<f8")
+    ke = np.ascontiguousarray(ke, dtype="<f8")
+    n = (p + 1) ** 3 * 5 * nx * ny * nz
+    if u.size != n or not 0 <= step <= timesteps or ke.size > timesteps:
+        raise ValueError("checkpoint sizes do not match the run")
+    head = np.zeros((), dtype=_CKPT_HEAD)
+    head["magic"] = CKPT_MAGIC
+    head["ints"] = [p, nx, ny, nz, precision]
+    head["steps"] = [step, timesteps]
+    head["reals"] = [dt, tf, L, Re, M0, V0, Pr, *lo, *hi]
+    with open(filename, "wb") as f:
+        f.write(head.tobytes())
+        f.write(np.int64(u.size).tobytes())
+        f.write(u.tobytes())
+        f.write(np.int64(ke.size).tobytes())
+        f.write(ke.tobytes())
+
+
+def load_checkpoint(filename):
+    """Read a checkpoint written by save_checkpoint or the C++ Solver; raises
+    ValueError on a malformed file."""
+    with open(filename, "rb") as f:
+        raw = f.read()
+    if len(raw) < _CKPT_HEAD.itemsize + 16 or raw[:8] != CKPT_MAGIC:
+        raise ValueError(f"{filename}: not a checkpoint")
+    head = np.frombuffer(raw[:_CKPT_HEAD.itemsize], dtype=_CKPT_HEAD)[0]
+    p, nx, ny, nz, precision = (int(v) for v in head["ints"])
+    step, timesteps = (int(v) for v in head["steps"])
+    r = [float(v) for v in head["reals"]]
+    off = _CKPT_HEAD.itemsize
+    n = int(np.frombuffer(raw[off:off + 8], dtype="<i8")[0])
+    if n != (p + 1) ** 3 * 5 * nx * ny * nz or len(raw) < off + 8 + 8 * n + 8:
+        raise ValueError(f"{filename}: state size does not match the header")
+    u = np.frombuffer(raw[off + 8:off + 8 + 8 * n], dtype="<f8").reshape(nx * ny * nz, 5, (p + 1) ** 3).copy()
+    off += 8 + 8 * n
+    m = int(np.frombuffer(raw[off:off + 8], dtype="<i8")[0])
+    if m < 0 or m > timesteps or len(raw) != off + 8 + 8 * m:
+        raise ValueError(f"{filename}: KE history size does not match the header")
+    ke = np.frombuffer(raw[off + 8:], dtype="<f8").copy()
+    return {"p": p, "nx": nx, "ny": ny, "nz": nz, "precision": precision, "step": step, "timesteps": timesteps,
+            "dt": r[0], "tf": r[1], "L": r[2], "Re": r[3], "M0": r[4], "V0": r[5], "Pr": r[6],
+            "lo": r[7:10], "hi": r[10:13], "u": u, "ke": ke}

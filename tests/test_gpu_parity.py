@@ -187,3 +187,70 @@ def test_cpp_driver_runs_and_writes_snapshots(tmp_path):
     vals = np.array([[float(v) for v in line.split(",")] for line in snap[1:]])
     np.testing.assert_allclose(vals[:, 3:].reshape(64, 27, 5), u0.transpose(0, 2, 1), rtol=1e-5, atol=1e-6)
     assert (out / "xyzu.txt.1").exists()
+
+
+def _ckpt_meta(prec, step, timesteps, dt, ke):
+    return dict(p=2, nx=4, ny=4, nz=4, precision=prec, step=step, timesteps=timesteps, dt=dt, tf=timesteps * dt,
+                L=1.0, Re=1600.0, M0=0.1, V0=1.0, Pr=0.71, lo=[-np.pi] * 3, hi=[np.pi] * 3, ke=ke)
+
+
+@pytest.mark.parametrize("prec", [ldg_amd.LDG_FP64, ldg_amd.LDG_MIXED])
+def test_checkpoint_resume_is_bitwise(tmp_path, prec):
+    """Stop after 2 of 6 steps, write a checkpoint, read it into a new solver
+    and continue: state and KE history equal the uninterrupted run bitwise."""
+    from ldg_amd import io
+    print(f"checkpoint resume precision={prec}", flush=True)
+    g = ldg_amd.LDG(2, 4, precision=prec)
+    g.initial_condition()
+    dt = g.time_step(0.01)
+    ke_all = g.step(dt, 6, record_ke=True)
+    u_all = g.get_state()
+    g.close()
+    g = ldg_amd.LDG(2, 4, precision=prec)
+    g.initial_condition()
+    ke1 = g.step(dt, 2, record_ke=True)
+    io.save_checkpoint(tmp_path / "c.ckpt", g.get_state(), **_ckpt_meta(prec, 2, 6, dt, ke1))
+    g.close()
+    c = io.load_checkpoint(tmp_path / "c.ckpt")
+    g = ldg_amd.LDG(2, 4, precision=prec)
+    g.set_state(c["u"])
+    ke2 = g.step(c["dt"], c["timesteps"] - c["step"], record_ke=True)
+    assert np.array_equal(g.get_state(), u_all)
+    assert np.array_equal(np.concatenate([c["ke"], ke2]), ke_all)
+    g.close()
+
+
+def test_cpp_driver_resumes_a_checkpoint(tmp_path):
+    """The C++ Solver resumes a checkpoint written from Python at step 3 of
+    its own run and ends bitwise where the uninterrupted run ends."""
+    import os
+    import subprocess
+    from ldg_amd import io
+    tf = 0.02
+    g = ldg_amd.LDG(2, 4)
+    g.initial_condition()
+    dt0 = g.time_step(0.01)
+    n = int(np.ceil(tf / dt0))
+    dt = tf / n
+    ke_all = g.step(dt, n, record_ke=True)
+    u_all = g.get_state()
+    g.close()
+    g = ldg_amd.LDG(2, 4)
+    g.initial_condition()
+    ke1 = g.step(dt, 3, record_ke=True)
+    meta = _ckpt_meta(ldg_amd.LDG_FP64, 3, n, dt, ke1)
+    meta["tf"] = tf
+    io.save_checkpoint(tmp_path / "mid.ckpt", g.get_state(), **meta)
+    g.close()
+    exe = os.path.join(os.path.dirname(ldg_amd.LIB_PATH), "ldg_driver")
+    r = subprocess.run([exe, "-p", "2", "-nx", "4", "-tf", str(tf), "-nosnap", "-quiet", "-resume", str(tmp_path / "mid.ckpt"),
+                        "-checkpoint", str(tmp_path / "end.ckpt")], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    c = io.load_checkpoint(tmp_path / "end.ckpt")
+    assert c["step"] == n and c["timesteps"] == n and c["dt"] == dt
+    assert np.array_equal(c["u"], u_all)
+    assert np.array_equal(c["ke"], ke_all)
+    # a checkpoint from another run is refused
+    r = subprocess.run([exe, "-p", "2", "-nx", "4", "-tf", str(2 * tf), "-nosnap", "-quiet", "-resume", str(tmp_path / "mid.ckpt")],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 1 and "different run" in r.stderr
