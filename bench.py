@@ -67,18 +67,43 @@ def alg_bytes(dof, mixed):
     return grad, rhs
 
 
-def profiled_traffic(kernel, precision, p, nx):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 summary
-    (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the copy-kernel calibration); only
-    for the 64^3 p=4 workload it was measured on, else None."""
+def profiled_value(key, kernel, precision, p, nx):
+    """A per-launch figure of `kernel` from the committed rocprofv3 summary
+    (machine-readable lines "key kernel precision value"): "traffic" = HBM
+    bytes (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the copy-kernel calibration),
+    "valu_insts" = wave64 vector instructions (SQ_INSTS_VALU); averaged over
+    the 4 RK stages, and only for the 64^3 p=4 workload they were measured
+    on, else None."""
     path = os.path.join(ROOT, "profiles", "ldg_kernels_tgv64_p4.md")
     if (p, nx) != (4, 64) or not os.path.exists(path):
         return None
     for line in open(path):
         f = line.split()
-        if len(f) == 4 and f[0] == "traffic" and f[1] == kernel and f[2] == precision:
+        if len(f) == 4 and f[0] == key and f[1] == kernel and f[2] == precision:
             return float(f[3])
     return None
+
+
+def profiled_traffic(kernel, precision, p, nx):
+    return profiled_value("traffic", kernel, precision, p, nx)
+
+
+# MI355X: 256 CUs x 4 SIMDs; a wave64 vector instruction occupies a SIMD for
+# 4 cycles (16 lanes per cycle, no packed FP32); 2.4 GHz peak engine clock.
+SIMDS, VALU_CYCLES, CLOCK_HZ = 1024, 4, 2.4e9
+
+
+def issue_bound(kernel, precision, p, nx, avg_ms):
+    """Fraction of the vector-issue ceiling the kernel uses: its profiled
+    vector instructions per launch x 4 cycles / (1024 SIMDs x peak clock x
+    its measured launch time)."""
+    v = profiled_value("valu_insts", kernel, precision, p, nx)
+    if v is None or not avg_ms:
+        return None
+    return {"kernel": f"ldg_{kernel}_kernel", "valu_insts_per_launch": v, "cycles_per_inst": VALU_CYCLES,
+            "simds": SIMDS, "clock_ghz": CLOCK_HZ / 1e9, "avg_ms": avg_ms,
+            "busy_frac": v * VALU_CYCLES / (SIMDS * CLOCK_HZ * avg_ms * 1e-3),
+            "source": "profiles/ldg_kernels_tgv64_p4.md (rocprofv3 SQ_INSTS_VALU)"}
 
 
 def rel_l2(a, b):
@@ -316,6 +341,7 @@ def main():
                      "frac": ach_rhs / HBM_PEAK_GBS, "traffic": profiled_traffic("rhs", main_name, p, nx),
                      "traffic_source": "profiles/ldg_kernels_tgv64_p4.md (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0)",
                      "kernel": "ldg_rhs_kernel", "alg_bytes_per_launch": b_rhs, "avg_ms": r["kt"]["rhs_ms"]},
+        "issue_bound": issue_bound("rhs", main_name, p, nx, r["kt"]["rhs_ms"]),
         "kernels": {
             name: {"ldg_grad_kernel": {"avg_ms": v["kt"]["grad_ms"],
                                        "alg_GBs": alg_bytes(dof, name == "mixed")[0] / (v["kt"]["grad_ms"] * 1e-3) / 1e9},

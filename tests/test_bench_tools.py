@@ -52,3 +52,14 @@ def test_validate_compare_refuses_different_runs(tmp_path):
     except AssertionError:
         return
     raise AssertionError("runs of different length were compared")
+
+
+def test_profiled_figures_and_issue_bound():
+    # the committed profile carries the machine-readable lines bench.py reads
+    t = bench.profiled_traffic("rhs", "mixed", 4, 64)
+    assert t is not None and t > bench.alg_bytes(163840000, True)[1]     # measured >= algorithmic
+    assert bench.profiled_traffic("rhs", "mixed", 4, 32) is None            # other workloads: not measured
+    ib = bench.issue_bound("rhs", "mixed", 4, 64, 2.0)
+    v = ib["valu_insts_per_launch"]
+    assert abs(ib["busy_frac"] - v * 4 / (1024 * 2.4e9 * 2.0e-3)) < 1e-12
+    assert bench.issue_bound("rhs", "mixed", 4, 32, 2.0) is None
