@@ -589,7 +589,11 @@ __device__ inline void rhs_face_dir(const UT *const (*pU)[7], const T *const (*p
                                     T (*__restrict__ sbuf)[RhsLds<N, T>::BUF], T (*__restrict__ sr)[NSTATE][N * N * N],
                                     double brho, double bE) {
   using FI = FaceItems<N, E, RhsLds<N, T>::BUF>;
-  T *const sb = &sbuf[0][0];
+  // where two face buffers fit in an element's volume buffer (p >= 4), the
+  // directions alternate between them, so a direction's first phase need not
+  // wait for the previous direction's last reads
+  constexpr bool DB = 2 * RhsLds<N, T>::FB <= RhsLds<N, T>::BUF;
+  T *const sb = &sbuf[0][0] + (DB && (D & 1) ? RhsLds<N, T>::FB : 0);
   constexpr int NN = N * N * N, NF = N * N;
   constexpr int st_d = D == 0 ? 1 : D == 1 ? N : N * N;
   constexpr int ast = D == 0 ? N : 1;          // memory stride of the face coordinate a
@@ -713,7 +717,9 @@ __device__ inline void rhs_face_dir(const UT *const (*pU)[7], const T *const (*p
       }
     }
   }
-  __syncthreads();
+  // the next direction writes the other face buffer; the lifts of two
+  // directions (both update sr) stay separated by its four barriers
+  if (!DB || D == 2) __syncthreads();
 }
 
 // Work mapping of the volume-line loops (E elements x X = 20 fields x N^2
