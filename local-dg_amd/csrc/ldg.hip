@@ -406,12 +406,19 @@ int set_dt(ldg_handle *h, const double *dt) {
 
 // One RK4 step of every group (eager launches).  With ke, each group's
 // kinetic energies of u^{n+1} go to row *d_step of the KE history.
-int rk4_step(ldg_handle *h, bool ke) {
+// One RK4 step of every group.  In the mixed path without a closure, a step
+// that follows another one in the same call (in32) reads its stage-1 input
+// as the FP32 fluctuations the previous step's stage 4 left in ub (out32);
+// ub is free then (stage 4 reads ua) and is only rewritten by stage 2.  The
+// kernels round an FP64 stage-1 input to the same FP32 values, so results do
+// not depend on how steps are split into calls or graph replays.
+int rk4_step(ldg_handle *h, bool ke, bool in32, bool out32) {
   static const double b[4] = {1 / 6.0, 1 / 3.0, 1 / 3.0, 1 / 6.0};
   static const double a[3] = {0.5, 0.5, 1.0};
   int first = 0;
   for (Group &g : h->groups) {
-    StageIn U = state_in(g.u);
+    const bool carry = h->mixed && g.cl.net == nullptr;
+    StageIn U = carry && in32 ? StageIn{g.ub, true} : state_in(g.u);
     void *bufs[2] = {g.ua, g.ub};
     for (int s = 0; s < 4; ++s) {
       int rc = launch_grad(h, g, U);
@@ -427,6 +434,7 @@ int rk4_step(ldg_handle *h, bool ke) {
         st.ca = a[s];
       }
       if (s == 3 && ke) st.ke_part = g.ke_part;
+      if (s == 3 && carry && out32) st.next32 = (float *)g.ub;
       rc = closure_stage(h, g, U, st);
       if (rc) return rc;
       rc = launch_rhs(h, g, U, st);
@@ -473,8 +481,8 @@ int get_graph(ldg_handle *h, bool ke, hipGraphExec_t *out) {
   gs = ldg_handle::GraphSlot();
   hipGraph_t graph = nullptr;
   HIPCHECK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-  int rc = rk4_step(h, ke);
-  if (!rc) rc = rk4_step(h, ke);
+  int rc = rk4_step(h, ke, false, true);
+  if (!rc) rc = rk4_step(h, ke, true, false);
   hipError_t e = hipStreamEndCapture(h->stream, &graph);
   if (rc) { if (graph) (void)hipGraphDestroy(graph); return rc; }
   if (e != hipSuccess) return fail(h, LDG_EDEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
@@ -523,8 +531,8 @@ int step_impl(ldg_handle *h, const double *dt, int nsteps, double *ke) {
     if (rc) return rc;
     for (; done + 2 <= nsteps; done += 2) HIPCHECK(h, hipGraphLaunch(exec, h->stream));
   }
-  for (; done < nsteps; ++done) {
-    rc = rk4_step(h, ke != nullptr);
+  for (int i = 0; done < nsteps; ++done, ++i) {
+    rc = rk4_step(h, ke != nullptr, i > 0, done + 1 < nsteps);
     if (rc) return rc;
   }
   if (ke && nsteps > 0)

@@ -70,6 +70,8 @@ struct StageArgs {
   double *acc;        // running u^n + dt sum b_s k_s
   double *unext;      // next stage input u^n + dt a_s k_s
   float *unext32;     // or, in the mixed path, that input as FP32 fluctuations about the background
+  float *next32;      // stage 4 (mixed): the new running sum u^{n+1} as FP32 fluctuations, the
+                      // next step's stage-1 input when that step follows in the same call
   double *kout;       // raw k (parity / single RHS)
   double cb, ca;      // RK4 coefficients b_s, a_s (times each instance's dt)
   int acc_init;       // 1: acc = u0 + dt b k   0: acc += dt b k
@@ -413,15 +415,25 @@ __global__ void __launch_bounds__(NT, (GradWaves<N, E, TQ>::value)) ldg_grad_ker
       const int base = line_base<N>(d, ab % N, ab / N), st = line_stride<N>(d);
       vj[it] = U[snb[e][2 * d] * (NSTATE * NN) + (s * NN + base + (N - 1) * st)];
     }
+    // an FP64 input in the mixed path (RK stage 1) is rounded to the FP32
+    // fluctuation the other stages read, so a step's result does not depend
+    // on which form its input came in
+    constexpr bool ROUND = sizeof(TQ) == 4 && sizeof(UT) == 8;
+    const double brho = ROUND ? d.ph.rho0 : 0.0, bE = ROUND ? d.ph.E0 : 0.0;
+    auto in = [&](UT x, int s) -> double {
+      if (!ROUND) return (double)x;
+      const double b = s == 0 ? brho : s == 4 ? bE : 0.0;
+      return (double)(float)((double)x - b);
+    };
 #pragma unroll
     for (int it = 0; it < UL; ++it) {
       const int w = tid + it * NT;
-      if (w < tot) (&su[0][0][0])[w] = (double)v[it];
+      if (w < tot) (&su[0][0][0])[w] = in(v[it], (w / NN) % NSTATE);
     }
 #pragma unroll
     for (int it = 0; it < JL; ++it) {
       const int w = tid + it * NT;
-      if (w < totj) (&sj[0][0][0][0])[w] = (double)vj[it];
+      if (w < totj) (&sj[0][0][0][0])[w] = in(vj[it], (w / NF) % NSTATE);
     }
   }
   __syncthreads();
@@ -1130,6 +1142,10 @@ ldg_rhs_kernel(const UT *__restrict__ U, const T *__restrict__ Q, LaunchArgs la,
           const double a = (st.acc_init ? ru[it] : ra[it]) + dtb * k;
           st.acc[gi] = a;
           if (st.ke_part) md(e)[r] = a;
+          if (st.next32) {
+            const int sf = r / NN;
+            st.next32[gi] = (float)(a - (sf == 0 ? ph.rho0 : sf == 4 ? ph.E0 : 0.0));
+          }
         }
         if (st.unext) st.unext[gi] = ru[it] + dta * k;
         if (st.unext32) {

@@ -68,3 +68,27 @@ def test_mixed_diagnostics_and_conservation():
     # FP32 fluxes: mass is conserved to FP32 round-off of the face fluxes
     assert abs(mass1 - mass0) <= 1e-9 * mass0
     assert abs(d0["ke"] - 0.1248) < 2e-3 and abs(d0["enstrophy"] - 0.3745) < 5e-3
+
+
+def test_mixed_steps_do_not_depend_on_call_split():
+    """Steps after the first in one call read their stage-1 input as the FP32
+    copy the previous step left; the first step of a call rounds the FP64
+    state to the same values.  Five steps in one call, one per call and by
+    graph replay (plus an eager remainder) end bitwise equal."""
+    print("mixed call-split invariance p=4 6^3", flush=True)
+    runs = []
+    for mode in ("one call", "single steps", "graph"):
+        g = ldg_amd.LDG(4, 6, precision=ldg_amd.LDG_MIXED)
+        g.initial_condition()
+        dt = g.time_step(0.01)
+        if mode == "graph":
+            g.set_graph(True)
+        if mode == "single steps":
+            ke = np.concatenate([g.step(dt, 1, record_ke=True) for _ in range(5)])
+        else:
+            ke = g.step(dt, 5, record_ke=True)
+        runs.append((g.get_state(), ke))
+        g.close()
+    for u, ke in runs[1:]:
+        assert np.array_equal(u, runs[0][0])
+        assert np.array_equal(ke, runs[0][1])
