@@ -52,18 +52,23 @@ def cpu_baseline(p, ne, steps):
 
 
 def alg_bytes(dof, mixed):
-    """Algorithmic HBM bytes per launch, averaged over the 4 RK stages (u^n,
-    the running sum and, in the FP64 path, the stage inputs are 8-byte words;
-    the gradient Q is 3 fields per state variable, 4 bytes each when mixed;
-    in the mixed path the stage inputs of stages 2-4 are 4-byte
-    fluctuations):
+    """Algorithmic HBM bytes per launch, averaged over the 4 RK stages of a
+    step in a long run.  u^n and the running sum are 8-byte words, Q is 3
+    fields per state variable (8 or, mixed, 4 bytes), a stage input U_s is
+    8 bytes (FP64) or a 4-byte fluctuation (mixed: stage 1 reads the copy of
+    u^n that the previous step's stage 4 wrote).
       grad: read U_s, write Q
-      rhs : read U_s, Q, u^n; acc read+write; write U_{s+1} (stage 1 reads
-            U_s = u^n once and no acc, stage 4 writes no next input)."""
+      rhs : read U_s, Q and u^n; read (stages 2-4) and write the running
+            sum; write U_{s+1} (stages 1-3) or, mixed, the FP32 copy of
+            u^{n+1} (stage 4).  In the FP64 path stage 1's U_s is u^n
+            itself, read once."""
     q = 3 * (4 if mixed else 8)
-    u1, us = 8, (4 if mixed else 8)      # stage input: stage 1, stages 2-4 (read and written)
-    grad = ((u1 + q) + 3 * (us + q)) / 4 * dof
-    rhs = ((u1 + q + 8 + us) + (us + q + 8 + 16 + us) * 2 + (us + q + 8 + 16)) / 4 * dof
+    us = 4 if mixed else 8
+    grad = (us + q) * dof
+    s1 = (us + q + 8 if mixed else 8 + q) + 8 + us          # U_s, Q, (u^n), acc write, U_2
+    s23 = us + q + 8 + 16 + us                              # U_s, Q, u^n, acc read+write, U_{s+1}
+    s4 = us + q + 8 + 16 + (4 if mixed else 0)              # U_4, Q, u^n, acc read+write, (copy of u^{n+1})
+    rhs = (s1 + 2 * s23 + s4) / 4 * dof
     return grad, rhs
 
 

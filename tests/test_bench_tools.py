@@ -11,10 +11,10 @@ import bench  # noqa: E402
 
 
 def test_alg_bytes_per_dof():
-    # FP64: grad U 8 + Q 24; rhs 58 B stage average.  Mixed: FP32 stage
-    # inputs after stage 1 and an FP32 gradient.
+    # FP64: grad U 8 + Q 24; rhs (48 + 2 x 64 + 56) / 4 = 58 B.  Mixed: FP32
+    # stage inputs and gradient; rhs (36 + 2 x 44 + 44) / 4 = 42 B.
     assert bench.alg_bytes(1, False) == (32.0, 58.0)
-    assert bench.alg_bytes(1, True) == (17.0, 40.0)
+    assert bench.alg_bytes(1, True) == (16.0, 42.0)
 
 
 def _fingerprint(path, sample, ke, norms, dt=1e-3, steps=None, idx=None):
