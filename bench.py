@@ -308,7 +308,7 @@ def main():
         g, dt, wall, ke = timed_run(ldg_amd, p, nx, prec, args.steps, args.warmup)
         dof = g.K * g.dofs * 5
         u = g.get_state()
-        kt = g.kernel_times(dt, 2)          # per-kernel device time, HIP events on the library's stream
+        kt = g.kernel_times(dt, 4)          # per-kernel device time, HIP events on the library's stream
         runs[name] = dict(K=g.K, dt=dt, wall=wall, ke=ke, dof=dof, kt=kt, u=u)
         g.close()
 

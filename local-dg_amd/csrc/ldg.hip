@@ -900,8 +900,11 @@ int ldg_kernel_times(ldg_handle *h, double dt, int nsteps, double ms[2]) {
   for (auto &e : ev) HIPCHECK(h, hipEventCreate(&e));
   double tot[2] = {0, 0};
   int count = 0;
+  // the same stage inputs as ldg_step: from the second step on, stage 1
+  // reads the FP32 copy of u^n the previous stage 4 wrote (mixed path)
+  const bool carry = h->mixed && g.cl.net == nullptr;
   for (int it = 0; it < nsteps; ++it) {
-    StageIn U = state_in(g.u);
+    StageIn U = carry && it > 0 ? StageIn{g.ub, true} : state_in(g.u);
     void *bufs[2] = {g.ua, g.ub};
     for (int s = 0; s < 4; ++s) {
       HIPCHECK(h, hipEventRecord(ev[0], h->stream));
@@ -915,6 +918,7 @@ int ldg_kernel_times(ldg_handle *h, double dt, int nsteps, double ms[2]) {
         else st.unext = (double *)bufs[s & 1];
         st.ca = a[s];
       }
+      if (s == 3 && carry && it + 1 < nsteps) st.next32 = (float *)g.ub;
       rc = launch_rhs(h, g, U, st);
       if (rc) return rc;
       HIPCHECK(h, hipEventRecord(ev[2], h->stream));
