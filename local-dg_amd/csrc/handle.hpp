@@ -80,8 +80,10 @@ struct ldg_handle {
   struct GraphSlot {
     hipGraphExec_t exec = nullptr;
     bool ke = false;
+    bool first = false;            // first step reads the FP64 state (first replay of a call)
     const double *u = nullptr;     // groups[0].u when captured
-  } graphs[2];
+  } graphs[4];
+  int graph_next = 0;              // slot replaced next when all are taken
   std::string err;
 };
 

@@ -12,8 +12,8 @@
 //          the mixed mode)
 // Steps can be captured into a HIP graph: two steps per graph, so that the
 // u/acc swap returns the arrays to their starting roles.  An odd number of
-// eager steps leaves the roles swapped, so up to two graphs are kept, one
-// per arrangement of the arrays.
+// eager steps leaves the roles swapped, so graphs are kept per arrangement of
+// the arrays and per kind of first step (FP64 state or FP32 carried input).
 #include "ldg.h"
 
 #include <hip/hip_runtime.h>
@@ -467,22 +467,32 @@ void drop_graph(ldg_handle *h) {
 
 // The graph for the current arrangement of the arrays, built on first use:
 // two eager steps recorded on the stream (the u/acc roles are the same after
-// two steps, so the graph can be replayed any number of times).
-int get_graph(ldg_handle *h, bool ke, hipGraphExec_t *out) {
+// two steps, so the graph can be replayed any number of times).  The first
+// replay of a call reads the FP64 state (first); later replays read the FP32
+// stage-1 input the previous replay left, like eager steps after the first.
+int get_graph(ldg_handle *h, bool ke, bool first, hipGraphExec_t *out) {
   const double *u = h->groups[0].u;
+  const int NS = (int)(sizeof(h->graphs) / sizeof(h->graphs[0]));
   int slot = -1;
-  for (int i = 0; i < 2; ++i)
-    if (h->graphs[i].exec && h->graphs[i].u == u && h->graphs[i].ke == ke) { *out = h->graphs[i].exec; return LDG_OK; }
-  for (int i = 0; i < 2 && slot < 0; ++i)
-    if (h->graphs[i].u == u || !h->graphs[i].exec) slot = i;
-  if (slot < 0) slot = 0;
+  for (int i = 0; i < NS; ++i) {
+    const auto &g = h->graphs[i];
+    if (g.exec && g.u == u && g.ke == ke && g.first == first) { *out = g.exec; return LDG_OK; }
+  }
+  for (int i = 0; i < NS && slot < 0; ++i)
+    if (!h->graphs[i].exec) slot = i;
+  // replace round-robin, but never the other variant of this arrangement:
+  // a call holds both at once
+  for (int k = 0; k < NS && slot < 0; ++k) {
+    const int i = h->graph_next++ % NS;
+    if (!(h->graphs[i].u == u && h->graphs[i].ke == ke)) slot = i;
+  }
   auto &gs = h->graphs[slot];
   if (gs.exec) (void)hipGraphExecDestroy(gs.exec);
   gs = ldg_handle::GraphSlot();
   hipGraph_t graph = nullptr;
   HIPCHECK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-  int rc = rk4_step(h, ke, false, true);
-  if (!rc) rc = rk4_step(h, ke, true, false);
+  int rc = rk4_step(h, ke, !first, true);
+  if (!rc) rc = rk4_step(h, ke, true, true);
   hipError_t e = hipStreamEndCapture(h->stream, &graph);
   if (rc) { if (graph) (void)hipGraphDestroy(graph); return rc; }
   if (e != hipSuccess) return fail(h, LDG_EDEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
@@ -494,6 +504,7 @@ int get_graph(ldg_handle *h, bool ke, hipGraphExec_t *out) {
   // were swapped twice, i.e. they are back where they started
   gs.exec = exec;
   gs.ke = ke;
+  gs.first = first;
   gs.u = u;
   *out = exec;
   return LDG_OK;
@@ -526,13 +537,14 @@ int step_impl(ldg_handle *h, const double *dt, int nsteps, double *ke) {
   HIPCHECK(h, hipMemcpyAsync(h->d_step, &zero, sizeof(int), hipMemcpyHostToDevice, h->stream));
   int done = 0;
   if (h->use_graph && nsteps >= 2) {
-    hipGraphExec_t exec = nullptr;
-    rc = get_graph(h, ke != nullptr, &exec);
+    hipGraphExec_t first = nullptr, later = nullptr;
+    rc = get_graph(h, ke != nullptr, true, &first);
+    if (!rc && nsteps >= 4) rc = get_graph(h, ke != nullptr, false, &later);
     if (rc) return rc;
-    for (; done + 2 <= nsteps; done += 2) HIPCHECK(h, hipGraphLaunch(exec, h->stream));
+    for (; done + 2 <= nsteps; done += 2) HIPCHECK(h, hipGraphLaunch(done ? later : first, h->stream));
   }
-  for (int i = 0; done < nsteps; ++done, ++i) {
-    rc = rk4_step(h, ke != nullptr, i > 0, done + 1 < nsteps);
+  for (; done < nsteps; ++done) {
+    rc = rk4_step(h, ke != nullptr, done > 0, done + 1 < nsteps);
     if (rc) return rc;
   }
   if (ke && nsteps > 0)
