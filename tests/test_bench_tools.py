@@ -63,3 +63,18 @@ def test_profiled_figures_and_issue_bound():
     v = ib["valu_insts_per_launch"]
     assert abs(ib["busy_frac"] - v * 4 / (1024 * 2.4e9 * 2.0e-3)) < 1e-12
     assert bench.issue_bound("rhs", "mixed", 4, 32, 2.0) is None
+
+
+def test_momentum_error_is_joint_over_components():
+    # only rho w differs, and rho w is small: its own relative error is large,
+    # the error relative to the whole momentum is not
+    rng = np.random.default_rng(2)
+    ref = rng.standard_normal((8, 5, 27))
+    ref[:, 3, :] *= 1e-4
+    mix = ref.copy()
+    mix[:, 3, :] += 1e-8 * rng.standard_normal((8, 27))
+    per = bench.rel_l2(mix, ref)
+    joint = bench.rel_l2_momentum(mix, ref)
+    direct = np.linalg.norm(mix[:, 3, :] - ref[:, 3, :]) / np.linalg.norm(ref[:, 1:4, :])
+    assert abs(joint - direct) <= 1e-12 * direct
+    assert per[3] > 1e2 * joint
