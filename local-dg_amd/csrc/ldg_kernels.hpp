@@ -484,6 +484,12 @@ struct RhsLds {
   static constexpr int SV = NV * NN;
   static constexpr int FB = 2 * NFF * NF;
   static constexpr int BUF = SV > FB ? SV : FB;      // per element, even for every N
+  // FP64 at p >= 3: each direction's face gather is issued during the
+  // previous direction (registers are free there: occupancy is bound by
+  // LDS), rhs -1 % at p = 3 and 4, -2 % at p = 5; at p = 2 it was 1 % slower
+  // and at p = 1 it spills.  FP32: the extra registers cost a wave per SIMD
+  // at p <= 3 (+7-13 %) and gain nothing at p >= 4.
+  static constexpr bool PREFETCH_FACES = sizeof(T) == 8 && N >= 4;
   // occupancy target (waves per SIMD) handed to the register allocator
   static constexpr int WAVES = sizeof(T) == 4 ? (N <= 5 ? 4 : 2) : (N <= 2 ? 4 : N <= 5 ? 3 : 2);
 };
@@ -592,15 +598,58 @@ struct FaceItems {
   __device__ inline int f(int it) const { return KEEP ? fk[it] : fcode(tid + it * NT, ne); }
 };
 
+// Gather the nodal face data of direction D for the block straight from HBM
+// into registers: own and neighbour traces and the gradient trace chosen by
+// the one-sided switch, one line along the face coordinate a per item.
+template <int N, int E, typename T, int D, typename UT>
+__device__ inline void face_gather(const UT *const (*pU)[7], const T *const (*pQ)[7],
+                                   const FaceItems<N, E, RhsLds<N, T>::BUF> &fi,
+                                   T (&v)[FaceItems<N, E, RhsLds<N, T>::BUF>::GL][N], double brho, double bE) {
+  using FI = FaceItems<N, E, RhsLds<N, T>::BUF>;
+  constexpr int NN = N * N * N;
+  constexpr int st_d = D == 0 ? 1 : D == 1 ? N : N * N;
+  constexpr int ast = D == 0 ? N : 1;          // memory stride of the face coordinate a
+  constexpr int bst = D == 2 ? N : N * N;      // memory stride of the face coordinate b
+#pragma unroll
+  for (int it = 0; it < FI::GL; ++it) {
+    const int gi = fi.g(it);
+    if (gi >= 0) {
+      const int e = gi >> 22, side = (gi >> 16) & 1, k = (gi >> 18) & 15, b = (gi >> 13) & 7;
+      if (!(gi & (1 << 17))) {
+        // own trace on this face, or the neighbour's on the opposite face
+        const bool nbr = k >= NSTATE;
+        const int s = nbr ? k - NSTATE : k;
+        const int at = (side ^ nbr) ? (N - 1) * st_d : 0;
+        const UT *src = pU[e][nbr ? 1 + 2 * D + side : 0] + (s * NN + at + b * bst);
+        const double bgs = s == 0 ? brho : s == 4 ? bE : 0.0;
+#pragma unroll
+        for (int a = 0; a < N; ++a) v[it][a] = T((double)src[a * ast] - bgs);
+      } else {
+        // own i_d = 0 slice (side 0) or the +d neighbour's (side 1)
+        const T *src = pQ[e][side ? 2 + 2 * D : 0] + (k * NN + b * bst);
+#pragma unroll
+        for (int a = 0; a < N; ++a) v[it][a] = src[a * ast];
+      }
+    } else {
+      // defined for every item, so that no earlier contents stay live
+#pragma unroll
+      for (int a = 0; a < N; ++a) v[it][a] = T(0);
+    }
+  }
+}
+
 // Face terms of one direction D (both faces of every element in the block):
-// gather nodal face data from HBM (own and neighbour traces, and the
-// gradient trace chosen by the one-sided switch), interpolate to the face
-// Gauss points, evaluate Roe + viscous fluxes, lift them back to the face
-// nodes and subtract them from the residual sr.  pU / pQ: per element, its
+// gather the face data into v (or, with PREFETCH_FACES, take what the
+// previous direction gathered there), interpolate to the face Gauss points,
+// evaluate Roe + viscous fluxes, lift them back to the face nodes and
+// subtract them from the residual sr.  With PREFETCH_FACES, the gather of
+// the next direction is issued into v once it is consumed, so its latency is
+// covered by this direction's remaining phases.  pU / pQ: per element, its
 // own (0) and its face neighbours' (1 + face) state and gradient.
 template <int N, int E, typename T, int D, typename UT>
 __device__ inline void rhs_face_dir(const UT *const (*pU)[7], const T *const (*pQ)[7],
                                     const FaceItems<N, E, RhsLds<N, T>::BUF> &fi,
+                                    T (&v)[FaceItems<N, E, RhsLds<N, T>::BUF>::GL][N],
                                     const MeshArgs &m, const SOps<N, T> *__restrict__ so, const Bg<T> &bg,
                                     T (*__restrict__ sbuf)[RhsLds<N, T>::BUF], T (*__restrict__ sr)[NSTATE][N * N * N],
                                     double brho, double bE) {
@@ -610,36 +659,17 @@ __device__ inline void rhs_face_dir(const UT *const (*pU)[7], const T *const (*p
   // wait for the previous direction's last reads
   constexpr bool DB = 2 * RhsLds<N, T>::FB <= RhsLds<N, T>::BUF;
   T *const sb = &sbuf[0][0] + (DB && (D & 1) ? RhsLds<N, T>::FB : 0);
-  constexpr int NN = N * N * N, NF = N * N;
+  constexpr int NF = N * N;
   constexpr int st_d = D == 0 ? 1 : D == 1 ? N : N * N;
   constexpr int ast = D == 0 ? N : 1;          // memory stride of the face coordinate a
   constexpr int bst = D == 2 ? N : N * N;      // memory stride of the face coordinate b
-  // gather nodal face data straight from HBM (all loads first) and
-  // interpolate along a: fb[side][f][qa + N b]; f = uK(0..4), uN(5..9), q~(10..24)
+  constexpr bool PF = RhsLds<N, T>::PREFETCH_FACES;
+  T vloc[FI::GL][N];
+  auto &vd = *(PF ? &v : &vloc);       // this direction's gathered lines
+  if constexpr (!PF) face_gather<N, E, T, D, UT>(pU, pQ, fi, vd, brho, bE);
+  // interpolate the gathered lines along a: fb[side][f][qa + N b];
+  // f = uK(0..4), uN(5..9), q~(10..24)
   {
-    T v[FI::GL][N];
-#pragma unroll
-    for (int it = 0; it < FI::GL; ++it) {
-      const int gi = fi.g(it);
-      if (gi >= 0) {
-        const int e = gi >> 22, side = (gi >> 16) & 1, k = (gi >> 18) & 15, b = (gi >> 13) & 7;
-        if (!(gi & (1 << 17))) {
-          // own trace on this face, or the neighbour's on the opposite face
-          const bool nbr = k >= NSTATE;
-          const int s = nbr ? k - NSTATE : k;
-          const int at = (side ^ nbr) ? (N - 1) * st_d : 0;
-          const UT *src = pU[e][nbr ? 1 + 2 * D + side : 0] + (s * NN + at + b * bst);
-          const double bgs = s == 0 ? brho : s == 4 ? bE : 0.0;
-#pragma unroll
-          for (int a = 0; a < N; ++a) v[it][a] = T((double)src[a * ast] - bgs);
-        } else {
-          // own i_d = 0 slice (side 0) or the +d neighbour's (side 1)
-          const T *src = pQ[e][side ? 2 + 2 * D : 0] + (k * NN + b * bst);
-#pragma unroll
-          for (int a = 0; a < N; ++a) v[it][a] = src[a * ast];
-        }
-      }
-    }
     SymReg<N, T> I;
     I.load(so->I);
 #pragma unroll
@@ -648,12 +678,13 @@ __device__ inline void rhs_face_dir(const UT *const (*pU)[7], const T *const (*p
       if (gi >= 0) {
         T *dst = sb + (gi & 8191);
         T o[N];
-        sym_apply<N, 1, T>(I.m, v[it], o);
+        sym_apply<N, 1, T>(I.m, vd[it], o);
 #pragma unroll
         for (int q = 0; q < N; ++q) dst[q] = o[q];
       }
     }
   }
+  if constexpr (PF && D < 2) face_gather<N, E, T, D + 1, UT>(pU, pQ, fi, v, brho, bE);
   __syncthreads();
   // interpolate along b (in place)
   {
@@ -818,9 +849,13 @@ ldg_rhs_kernel(const UT *__restrict__ U, const T *__restrict__ Q, LaunchArgs la,
   __syncthreads();
 
   // ================= faces, one direction at a time =================
-  rhs_face_dir<N, E, T, 0, UT>(pU, pQ, fi, m, so, bg, sbuf, sr, brho, bE);
-  rhs_face_dir<N, E, T, 1, UT>(pU, pQ, fi, m, so, bg, sbuf, sr, brho, bE);
-  rhs_face_dir<N, E, T, 2, UT>(pU, pQ, fi, m, so, bg, sbuf, sr, brho, bE);
+  {
+    T fv[FaceItems<N, E, BUF>::GL][N];
+    if constexpr (L::PREFETCH_FACES) face_gather<N, E, T, 0, UT>(pU, pQ, fi, fv, brho, bE);
+    rhs_face_dir<N, E, T, 0, UT>(pU, pQ, fi, fv, m, so, bg, sbuf, sr, brho, bE);
+    rhs_face_dir<N, E, T, 1, UT>(pU, pQ, fi, fv, m, so, bg, sbuf, sr, brho, bE);
+    rhs_face_dir<N, E, T, 2, UT>(pU, pQ, fi, fv, m, so, bg, sbuf, sr, brho, bE);
+  }
 
   // ================= volume =================
   // load U (minus the background) and Q x-lines from HBM, all loads of the
@@ -1054,8 +1089,8 @@ ldg_rhs_kernel(const UT *__restrict__ U, const T *__restrict__ Q, LaunchArgs la,
   // are copied straight into the part of the volume buffer that is free from
   // here on (after the 5 FP64 result fields) by LDS-direct loads, so their
   // latency is covered by the inverse-mass sweeps below (1.5 % faster at
-  // 64^3, p = 4).  The FP32 layout has room for one of the two only, and
-  // there the copies cost more than they save: its epilogue loads directly.
+  // 64^3, p = 4).  The FP32 layout has room for one of the two only; there
+  // they are loaded into registers at the same point instead.
   const bool need_u0 = st.acc || st.unext || st.unext32;
   const bool need_acc = st.acc && !st.acc_init;
   constexpr bool PF = sizeof(T) == sizeof(double);
@@ -1079,6 +1114,20 @@ ldg_rhs_kernel(const UT *__restrict__ U, const T *__restrict__ Q, LaunchArgs la,
                                              4, 0, 0);
         }
       }
+    }
+  }
+  constexpr int EPI = (E * NSTATE * NN + NT - 1) / NT;
+  const int tot_epi = ne * NSTATE * NN;
+  const long gofs = k0 * NSTATE * NN;
+  double ru[EPI], ra[EPI];
+#pragma unroll
+  for (int it = 0; it < EPI; ++it) {
+    const int w = tid + it * NT;
+    ru[it] = 0.0;
+    ra[it] = 0.0;
+    if (!PF && w < tot_epi) {
+      if (need_u0) ru[it] = st.u0[gofs + w];
+      if (need_acc) ra[it] = st.acc[gofs + w];
     }
   }
   // M1^{-1} along y, then along z (FP64)
@@ -1115,20 +1164,16 @@ ldg_rhs_kernel(const UT *__restrict__ U, const T *__restrict__ Q, LaunchArgs la,
   // ---- epilogue (FP64): k, RK4 accumulation and next stage input.  All of
   // a thread's HBM reads are issued before any is used.
   {
-    constexpr int EPI = (E * NSTATE * NN + NT - 1) / NT;
-    const int tot = ne * NSTATE * NN;
-    const long gofs = k0 * NSTATE * NN;
-    double ru[EPI], ra[EPI], kv[EPI];
+    const int tot = tot_epi;
+    double kv[EPI];
 #pragma unroll
     for (int it = 0; it < EPI; ++it) {
       const int w = tid + it * NT;
-      ru[it] = 0.0;
-      ra[it] = 0.0;
       kv[it] = 0.0;
       if (w < tot) {
         const int e = w / (NSTATE * NN), r = w - e * (NSTATE * NN);
-        if (need_u0) ru[it] = PF ? pf_u0(e)[r] : st.u0[gofs + w];
-        if (need_acc) ra[it] = PF ? pf_acc(e)[r] : st.acc[gofs + w];
+        if (PF && need_u0) ru[it] = pf_u0(e)[r];
+        if (PF && need_acc) ra[it] = pf_acc(e)[r];
         kv[it] = md(e)[r];
         if (st.corr && st.gate[k0 + e]) kv[it] += st.corr[gofs + w];
       }
