@@ -51,7 +51,10 @@ def test_python_rejects_malformed(tmp_path):
 def test_cpp_and_python_agree(tmp_path):
     exe = tmp_path / "checkpoint_check"
     host = os.path.join(ROOT, "local-dg_amd", "host")
-    subprocess.run(["g++", "-std=c++17", "-O1", "-Wall", "-I" + host, os.path.join(ROOT, "tests", "checkpoint_check.cpp"),
+    # built with the address and undefined-behaviour sanitizers, so a read or
+    # write past a buffer in the reader/writer aborts the check
+    subprocess.run(["g++", "-std=c++17", "-O1", "-Wall", "-fsanitize=address,undefined", "-fno-sanitize-recover=all",
+                    "-fno-omit-frame-pointer", "-I" + host, os.path.join(ROOT, "tests", "checkpoint_check.cpp"),
                     os.path.join(host, "io.cpp"), "-o", str(exe)], check=True, capture_output=True, text=True)
     meta, u = _sample()
     # C++ writes, Python reads

@@ -18,7 +18,10 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
                     reason="needs g++ and the HIP headers")
 def test_symop_split_matches_plain_product(tmp_path):
     exe = tmp_path / "symop_check"
+    # host build with the address and undefined-behaviour sanitizers: an
+    # index past an operator's split arrays aborts the check
     cmd = ["g++", "-std=c++17", "-O1", "-Wall", "-Wno-unknown-pragmas", "-D__HIP_PLATFORM_AMD__",
+           "-fsanitize=address,undefined", "-fno-sanitize-recover=all", "-fno-omit-frame-pointer",
            "-I/opt/rocm/include", "-I" + os.path.join(ROOT, "local-dg_amd", "csrc"),
            os.path.join(ROOT, "tests", "symop_check.cpp"), "-o", str(exe)]
     subprocess.run(cmd, check=True, capture_output=True, text=True)
