@@ -348,10 +348,11 @@ __device__ inline double block_sum(double v, double *sw) {
 // consecutive nodes, coalesced).
 // ---------------------------------------------------------------------
 // Occupancy target of the gradient kernel (waves per SIMD): 6 with an FP32
-// gradient up to p = 4 (measured 0.79 vs 0.87 ms at 64^3, p = 4), 5 (the
-// compiler's own choice) with FP64 and at p = 5 (where 6 spills); never
-// more than its LDS footprint allows (one workgroup = one wave per SIMD,
-// 160 KiB of LDS per CU).
+// gradient up to p = 4 (measured with FP64 arithmetic: 0.79 vs 0.87 ms at
+// 64^3, p = 4; the FP32 arithmetic needs 58 registers, so 8 fit anyway), 5
+// (the compiler's own choice) with FP64 and at p = 5 (where 6 spilled);
+// never more than its LDS footprint allows (one workgroup = one wave per
+// SIMD, 160 KiB of LDS per CU).
 template <int N, int E, typename TQ>
 struct GradWaves {
   static constexpr int lds = (int)sizeof(TQ) * (E * NSTATE * N * N * N + E * 3 * NSTATE * N * N + N * (N + 1)) + 8 * 6 * E;
