@@ -863,6 +863,19 @@ int ldg_autotune(ldg_handle *h, int *chosen) {
   HIPCHECK(h, hipEventCreate(&e1));
   int gi = 0;
   for (Group &g : h->groups) {
+    // time the kernels on the stage input they read in a run: in the mixed
+    // path without a closure that is the FP32 fluctuation of the state,
+    // written into ub (free between calls) by one rhs launch with a_s = 0
+    const bool f32 = h->mixed && g.cl.net == nullptr;
+    if (f32) {
+      StageArgs st = empty_stage();
+      st.u0 = g.u;
+      st.unext32 = (float *)g.ub;
+      st.ca = 0.0;
+      const int rc = launch_rhs(h, g, state_in(g.u), st, g.crhs);
+      if (rc) return rc;
+    }
+    const StageIn in = f32 ? StageIn{g.ub, true} : state_in(g.u);
     for (int kind = 0; kind < 2; ++kind) {     // 0: gradient, 1: right-hand side
       float best = 1e30f;
       int bestc = 0;
@@ -873,11 +886,11 @@ int ldg_autotune(ldg_handle *h, int *chosen) {
           if (rep == 1) HIPCHECK(h, hipEventRecord(e0, h->stream));
           int rc;
           if (kind == 0) {
-            rc = launch_grad(h, g, state_in(g.u), c);
+            rc = launch_grad(h, g, in, c);
           } else {
             StageArgs st = empty_stage();
             st.kout = g.k;
-            rc = launch_rhs(h, g, state_in(g.u), st, c);
+            rc = launch_rhs(h, g, in, st, c);
           }
           if (rc) return rc;
         }
