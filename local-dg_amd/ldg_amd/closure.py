@@ -187,7 +187,7 @@ def gated(solver):
 
 
 # ---------------------------------------------------------------------------
-# training data from high-order runs, training, and a judged run
+# training data from high-order runs, training, and an evaluation run
 # ---------------------------------------------------------------------------
 
 def training_instances(runs, seed, ne=24, M0=0.1, re_range=(800.0, 3200.0)):
