@@ -98,18 +98,24 @@ def test_phase_shift_is_a_periodic_roll():
     assert np.abs(b - rolled).max() <= 1e-13 * np.abs(a).max()
 
 
-def test_graph_replay_matches_eager():
+@pytest.mark.parametrize("precision", [ldg_amd.LDG_FP64, ldg_amd.LDG_MIXED])
+def test_graph_replay_matches_eager(precision):
     """Steps replayed from HIP graphs equal eager launches bit for bit:
     odd step counts (an eager remainder that flips the array roles), a
     change of dt between calls, diagnostics between replays, with and without
-    the kinetic-energy record."""
-    eager = ldg_amd.Ensemble(MEMBERS[:3])
-    graph = ldg_amd.Ensemble(MEMBERS[:3])
+    the kinetic-energy record.  In the mixed path the first replay of a call
+    reads the FP64 state and later ones the FP32 copy the previous replay
+    left; the plan needs more graph variants than the cache holds, so
+    variants are evicted and rebuilt."""
+    print("graph vs eager, precision", precision, flush=True)
+    eager = ldg_amd.Ensemble(MEMBERS[:3], precision=precision)
+    graph = ldg_amd.Ensemble(MEMBERS[:3], precision=precision)
     graph.set_graph(True)
     for e in (eager, graph):
         e.initial_condition()
     dts = eager.time_steps(0.01)
-    plan = [(dts, 7, True), (dts, 4, True), (dts * 0.5, 5, True), (dts * 0.5, 6, False), (dts, 3, True)]
+    plan = [(dts, 7, True), (dts, 4, True), (dts * 0.5, 5, True), (dts * 0.5, 6, False), (dts, 3, True),
+            (dts, 2, False), (dts, 9, False), (dts, 8, True), (dts, 1, True), (dts, 6, True)]
     for call, (dt, n, rec) in enumerate(plan):
         ke_a = eager.step(dt, n, record_ke=rec)
         ke_b = graph.step(dt, n, record_ke=rec)
