@@ -590,6 +590,21 @@ ldg_instance default_instance(const ldg_config &c) {
   return in;
 }
 
+// HIP events destroyed on every way out of a timing function (ldg_autotune,
+// ldg_kernel_times).
+template <int K>
+struct EventSet {
+  hipEvent_t e[K] = {};
+  int n = 0;
+  ~EventSet() {
+    for (int i = 0; i < n; ++i) (void)hipEventDestroy(e[i]);
+  }
+  int create(ldg_handle *h) {
+    for (; n < K; ++n) HIPCHECK(h, hipEventCreate(&e[n]));
+    return LDG_OK;
+  }
+};
+
 }  // namespace ldg_host
 
 using namespace ldg_host;
@@ -858,9 +873,9 @@ int ldg_diagnostics(ldg_handle *h, double out[3]) {
 int ldg_autotune(ldg_handle *h, int *chosen) {
   if (!h) return LDG_EINVAL;
   drop_graph(h);
-  hipEvent_t e0, e1;
-  HIPCHECK(h, hipEventCreate(&e0));
-  HIPCHECK(h, hipEventCreate(&e1));
+  EventSet<2> evs;
+  if (int rc = evs.create(h)) return rc;
+  hipEvent_t e0 = evs.e[0], e1 = evs.e[1];
   int gi = 0;
   for (Group &g : h->groups) {
     // time the kernels on the stage input they read in a run: in the mixed
@@ -908,8 +923,6 @@ int ldg_autotune(ldg_handle *h, int *chosen) {
     }
     ++gi;
   }
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
   return LDG_OK;
 }
 
@@ -921,8 +934,9 @@ int ldg_kernel_times(ldg_handle *h, double dt, int nsteps, double ms[2]) {
   int rc = set_dt(h, &dt);
   if (rc) return rc;
   Group &g = h->groups[0];
-  hipEvent_t ev[3];
-  for (auto &e : ev) HIPCHECK(h, hipEventCreate(&e));
+  EventSet<3> evs;
+  if (int c = evs.create(h)) return c;
+  hipEvent_t *ev = evs.e;
   double tot[2] = {0, 0};
   int count = 0;
   // the same stage inputs as ldg_step: from the second step on, stage 1
@@ -956,7 +970,6 @@ int ldg_kernel_times(ldg_handle *h, double dt, int nsteps, double ms[2]) {
     }
     std::swap(g.u, g.acc);
   }
-  for (auto &e : ev) (void)hipEventDestroy(e);
   ms[0] = tot[0] / count;
   ms[1] = tot[1] / count;
   return LDG_OK;
