@@ -4,9 +4,10 @@
 //
 //   ldg_driver [-p 2] [-nx 10] [-L 1] [-tf 10] [-dtSnaps 30] [-cfl 0.01]
 //              [-Re 1600] [-M0 0.1] [-mixed] [-nosnap] [-out output] [-quiet]
-//              [-checkpoint FILE] [-resume FILE]
+//              [-checkpoint FILE] [-resume FILE] [-wallLimit SECONDS]
 // -checkpoint writes a binary checkpoint when the run ends or hits the wall
-// limit; -resume continues a run from one (same options otherwise).
+// limit; -resume continues a run from one (same options otherwise);
+// -wallLimit replaces the 19.5 h limit (the reference's, solver.cpp:351).
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -39,6 +40,7 @@ int main(int argc, char **argv) {
     else if (!std::strcmp(argv[i], "-quiet")) opt.verbose = false;
     else if (!std::strcmp(argv[i], "-checkpoint")) opt.checkpointFile = next("-checkpoint");
     else if (!std::strcmp(argv[i], "-resume")) resume = next("-resume");
+    else if (!std::strcmp(argv[i], "-wallLimit")) opt.wallLimitSeconds = std::atof(next("-wallLimit"));
     else { std::cerr << "unknown option " << argv[i] << std::endl; return 2; }
   }
   try {

@@ -98,7 +98,7 @@ void Solver::dgTimeStep() {
   long &iStep = step;
   while (iStep < timesteps) {
     const double elapsed = std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();
-    if (elapsed > opt.wallLimitSeconds) {     // dump state and stop (solver.cpp:349-360)
+    if (elapsed >= opt.wallLimitSeconds) {    // dump state and stop (solver.cpp:349-360)
       const std::string name = opt.outputDir + "/xyznavier.tmp";
       if (!initXYZVFile(name, 0, "u", 5) || !exportToXYZVFile(name, 0, coords, u()))
         throw std::runtime_error("could not write " + name);

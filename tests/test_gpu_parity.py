@@ -254,3 +254,37 @@ def test_cpp_driver_resumes_a_checkpoint(tmp_path):
     r = subprocess.run([exe, "-p", "2", "-nx", "4", "-tf", str(2 * tf), "-nosnap", "-quiet", "-resume", str(tmp_path / "mid.ckpt")],
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 1 and "different run" in r.stderr
+
+
+def test_cpp_driver_wall_limit_dumps_and_stops_cleanly(tmp_path):
+    """A wall limit that has already passed (-wallLimit 0) makes the C++
+    Solver dump the state (xyznavier.tmp.0, the reference's name plus the
+    snapshot index its XYZV writer appends) and a checkpoint before its first
+    step and return normally (the reference's 19.5 h guard, solver.cpp:349-360);
+    resuming from that checkpoint ends bitwise where an uninterrupted run
+    ends."""
+    import os
+    import subprocess
+    from ldg_amd import io
+    print("C++ driver wall limit", flush=True)
+    tf = 0.02
+    exe = os.path.join(os.path.dirname(ldg_amd.LIB_PATH), "ldg_driver")
+    out = tmp_path / "out"
+    r = subprocess.run([exe, "-p", "2", "-nx", "4", "-tf", str(tf), "-quiet", "-out", str(out), "-wallLimit", "0",
+                        "-checkpoint", str(tmp_path / "wall.ckpt")], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert (out / "xyznavier.tmp.0").stat().st_size > 0
+    c = io.load_checkpoint(tmp_path / "wall.ckpt")
+    assert c["step"] == 0
+    r = subprocess.run([exe, "-p", "2", "-nx", "4", "-tf", str(tf), "-nosnap", "-quiet", "-resume", str(tmp_path / "wall.ckpt"),
+                        "-checkpoint", str(tmp_path / "end.ckpt")], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    end = io.load_checkpoint(tmp_path / "end.ckpt")
+    g = ldg_amd.LDG(2, 4)
+    g.initial_condition()
+    ke_all = g.step(end["dt"], end["timesteps"], record_ke=True)
+    u_all = g.get_state()
+    g.close()
+    assert end["step"] == end["timesteps"]
+    assert np.array_equal(end["u"], u_all)
+    assert np.array_equal(end["ke"], ke_all)
