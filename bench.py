@@ -32,23 +32,51 @@ METRIC = "DOF-stage updates/s: mixed vs FP64 plain at stated error; -dEk/dt err"
 HBM_PEAK_GBS = 8000.0
 
 
-def cpu_baseline(p, ne, steps):
-    """Oracle (FP64, dense operators) on a bounded sample: `steps` RK4 steps
-    of the Taylor-Green state on an ne^3 mesh at order p."""
+def cpu_model():
+    """The host CPU's model name (/proc/cpuinfo), or what platform reports."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
+    return platform.processor() or "unknown"
+
+
+def cpu_baseline(p, ne, steps, ne1, steps1):
+    """Oracle (FP64, dense operators) on bounded samples of the workload:
+    `steps` RK4 steps of the Taylor-Green state on an ne^3 mesh at order p
+    with all OpenMP threads, and `steps1` steps on an ne1^3 mesh with one."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
-    o = O.Oracle(p, ne)
-    u = o.initial_condition()
-    dt = o.dt(0.01)
-    o.step(u, dt, 1, record_ke=False)     # warm-up (first-touch, thread pool)
-    t = time.perf_counter()
-    o.step(u, dt, steps, record_ke=True)
-    wall = time.perf_counter() - t
-    dof = o.K * o.dofs * 5
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": 4 * steps * dof / wall, "unit": "DOF-stage updates/s", "cores": cores, "kind": "port",
+    lib = O.lib()
+    threads = lib.orc_max_threads()
+
+    def timed(ne_, steps_):
+        o = O.Oracle(p, ne_)
+        u = o.initial_condition()
+        dt = o.dt(0.01)
+        o.step(u, dt, 1, record_ke=False)     # warm-up (first-touch, thread pool)
+        t = time.perf_counter()
+        o.step(u, dt, steps_, record_ke=True)
+        wall = time.perf_counter() - t
+        return 4 * steps_ * o.K * o.dofs * 5 / wall, wall
+
+    value, wall = timed(ne, steps)
+    lib.orc_set_threads(1)
+    try:
+        value1, wall1 = timed(ne1, steps1)
+    finally:
+        lib.orc_set_threads(threads)
+    return {"value": value, "unit": "DOF-stage updates/s", "cores": threads, "kind": "port",
             "sample": f"oracle FP64 dense, Taylor-Green {ne}^3 elements p={p}, {steps} RK4 steps after 1 warm-up, "
-                      f"{wall:.1f} s"}
+                      f"{threads} OpenMP threads, {wall:.1f} s",
+            "cpu_model": cpu_model(), "host_cpus": os.cpu_count(),
+            "one_thread": {"value": value1, "unit": "DOF-stage updates/s",
+                           "sample": f"Taylor-Green {ne1}^3 elements p={p}, {steps1} RK4 step(s), 1 thread, "
+                                     f"{wall1:.1f} s"}}
 
 
 def alg_bytes(dof, mixed):
@@ -72,19 +100,38 @@ def alg_bytes(dof, mixed):
     return grad, rhs
 
 
-def profiled_value(key, kernel, precision, p, nx):
+# the device code the committed rocprofv3 figures were measured on
+KERNEL_SOURCES = ("local-dg_amd/csrc/ldg_kernels.hpp", "local-dg_amd/csrc/symop.hpp")
+
+
+def kernel_source_hash():
+    """First 12 hex digits of the SHA-256 of the kernel sources."""
+    import hashlib
+    h = hashlib.sha256()
+    for rel in KERNEL_SOURCES:
+        with open(os.path.join(ROOT, rel), "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()[:12]
+
+
+PROFILE = os.path.join(ROOT, "profiles", "ldg_kernels_tgv64_p4.md")
+
+
+def profiled_value(key, kernel, precision, p, nx, path=None):
     """A per-launch figure of `kernel` from the committed rocprofv3 summary
-    (machine-readable lines "key kernel precision value"): "traffic" = HBM
-    bytes (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the copy-kernel calibration),
-    "valu_insts" = wave64 vector instructions (SQ_INSTS_VALU); averaged over
-    the 4 RK stages, and only for the 64^3 p=4 workload they were measured
-    on, else None."""
-    path = os.path.join(ROOT, "profiles", "ldg_kernels_tgv64_p4.md")
+    (machine-readable lines "key kernel precision value src=HASH"):
+    "traffic" = HBM bytes (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the
+    copy-kernel calibration), "valu_insts" = wave64 vector instructions
+    (SQ_INSTS_VALU); averaged over the 4 RK stages.  None unless the
+    workload is the 64^3 p=4 one they were measured on and HASH is the
+    current kernel_source_hash() (a figure of other kernel code is stale)."""
+    path = path or PROFILE
     if (p, nx) != (4, 64) or not os.path.exists(path):
         return None
+    cur = "src=" + kernel_source_hash()
     for line in open(path):
         f = line.split()
-        if len(f) == 4 and f[0] == key and f[1] == kernel and f[2] == precision:
+        if len(f) == 5 and f[0] == key and f[1] == kernel and f[2] == precision and f[4] == cur:
             return float(f[3])
     return None
 
@@ -234,7 +281,9 @@ def main():
                     help="path reported as `value` (both are timed)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-nx", type=int, default=32)
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu1-nx", type=int, default=12, help="mesh of the one-thread CPU sample")
+    ap.add_argument("--cpu1-steps", type=int, default=1)
     ap.add_argument("--validate", type=float, default=0.0, metavar="T",
                     help="run both paths to time T and print their difference only")
     ap.add_argument("--validate-part", choices=["fp64", "mixed"], default=None,
@@ -365,7 +414,7 @@ def main():
             for name, v in runs.items()},
     }
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(p, args.cpu_nx, args.cpu_steps)
+        out["cpu_baseline"] = cpu_baseline(p, args.cpu_nx, args.cpu_steps, args.cpu1_nx, args.cpu1_steps)
     print(json.dumps(out), flush=True)
 
 

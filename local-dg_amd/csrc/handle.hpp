@@ -20,6 +20,9 @@ using namespace ldg;
 
 // Candidate tilings (elements per workgroup) per kernel and order.
 constexpr int kMaxCand = 2;
+// steps of kinetic-energy history held on the device (per instance); a call
+// recording more steps copies the history out in chunks of this many
+constexpr int kKeHistSteps = 1024;
 
 struct Inst {
   int p, nx, ny, nz;
@@ -74,7 +77,7 @@ struct ldg_handle {
   hipStream_t stream = nullptr;
   double *ke_hist = nullptr, *scal = nullptr;
   int *d_step = nullptr;
-  size_t ke_hist_cap = 0;
+  size_t ke_hist_cap = 0;          // kKeHistSteps x instances, allocated at create
   std::vector<double> dt_cur;       // dt currently written into the instance tables
   bool use_graph = false;
   struct GraphSlot {

@@ -375,8 +375,12 @@ int create_impl(const ldg_config *cfg, const ldg_instance *insts, int n, ldg_han
       return bad(LDG_EDEVICE);
     if (hipMemsetAsync(g.u, 0, bytes, h->stream) != hipSuccess) { h->err = "device initialisation failed"; return bad(LDG_EDEVICE); }
   }
+  // the device KE history holds kKeHistSteps steps; longer calls copy it
+  // out chunk by chunk, so a step call never allocates
+  h->ke_hist_cap = (size_t)kKeHistSteps * n;
   if (hipMalloc((void **)&h->scal, sizeof(double) * 3 * n) != hipSuccess ||
-      hipMalloc((void **)&h->d_step, sizeof(int)) != hipSuccess) {
+      hipMalloc((void **)&h->d_step, sizeof(int)) != hipSuccess ||
+      hipMalloc((void **)&h->ke_hist, sizeof(double) * h->ke_hist_cap) != hipSuccess) {
     h->err = "device allocation failed (scalars)";
     return bad(LDG_ENOMEM);
   }
@@ -523,32 +527,35 @@ int step_impl(ldg_handle *h, const double *dt, int nsteps, double *ke) {
   for (size_t i = 0; i < h->inst.size(); ++i)
     if (!(dt[i] > 0)) return fail(h, LDG_EINVAL, "bad step arguments");
   const size_t ni = h->inst.size();
-  if (ke && (size_t)nsteps * ni > h->ke_hist_cap) {
-    if (h->ke_hist) (void)hipFree(h->ke_hist);
-    h->ke_hist = nullptr;
-    h->ke_hist_cap = 0;
-    HIPCHECK(h, hipMalloc((void **)&h->ke_hist, sizeof(double) * nsteps * ni));
-    h->ke_hist_cap = (size_t)nsteps * ni;
-    drop_graph(h);            // the graph captured the old history pointer
-  }
   int rc = set_dt(h, dt);
   if (rc) return rc;
+  // chunks of at most kKeHistSteps steps when the KE history is recorded:
+  // the device history row counter restarts and the rows are copied out
+  // after each chunk
   const int zero = 0;
-  HIPCHECK(h, hipMemcpyAsync(h->d_step, &zero, sizeof(int), hipMemcpyHostToDevice, h->stream));
   int done = 0;
-  if (h->use_graph && nsteps >= 2) {
-    hipGraphExec_t first = nullptr, later = nullptr;
-    rc = get_graph(h, ke != nullptr, true, &first);
-    if (!rc && nsteps >= 4) rc = get_graph(h, ke != nullptr, false, &later);
-    if (rc) return rc;
-    for (; done + 2 <= nsteps; done += 2) HIPCHECK(h, hipGraphLaunch(done ? later : first, h->stream));
+  while (done < nsteps) {
+    const int n = ke ? std::min<int>((int)(h->ke_hist_cap / ni), nsteps - done) : nsteps - done;
+    if (ke) HIPCHECK(h, hipMemcpyAsync(h->d_step, &zero, sizeof(int), hipMemcpyHostToDevice, h->stream));
+    int k = 0;
+    if (h->use_graph && n >= 2) {
+      // the first replay of a call reads the FP64 state, later ones the FP32
+      // stage-1 input the step before left
+      hipGraphExec_t first = nullptr, later = nullptr;
+      if (done == 0) rc = get_graph(h, ke != nullptr, true, &first);
+      if (!rc && (done > 0 || n >= 4)) rc = get_graph(h, ke != nullptr, false, &later);
+      if (rc) return rc;
+      for (; k + 2 <= n; k += 2) HIPCHECK(h, hipGraphLaunch(done + k ? later : first, h->stream));
+    }
+    for (; k < n; ++k) {
+      rc = rk4_step(h, ke != nullptr, done + k > 0, done + k + 1 < nsteps);
+      if (rc) return rc;
+    }
+    if (ke)
+      HIPCHECK(h, hipMemcpyAsync(ke + (size_t)done * ni, h->ke_hist, sizeof(double) * n * ni, hipMemcpyDeviceToHost,
+                                 h->stream));
+    done += n;
   }
-  for (; done < nsteps; ++done) {
-    rc = rk4_step(h, ke != nullptr, done > 0, done + 1 < nsteps);
-    if (rc) return rc;
-  }
-  if (ke && nsteps > 0)
-    HIPCHECK(h, hipMemcpyAsync(ke, h->ke_hist, sizeof(double) * nsteps * ni, hipMemcpyDeviceToHost, h->stream));
   HIPCHECK(h, hipStreamSynchronize(h->stream));
   if (ke && ni > 1) {
     // the device history is numbered group by group; reorder each row to

@@ -14,6 +14,7 @@
 #include <iostream>
 
 #include "mesh.hpp"
+#include "mpiutil.hpp"
 #include "solver.hpp"
 
 int main(int argc, char **argv) {
@@ -44,8 +45,10 @@ int main(int argc, char **argv) {
     else { std::cerr << "unknown option " << argv[i] << std::endl; return 2; }
   }
   try {
+    ldg::MPIUtil mpi{};
+    if (mpi.rank == mpi.ROOT) std::cout << "MPI tasks = " << mpi.np << std::endl;
     const double size = M_PI * L;
-    ldg::Mesh mesh(nx, nx, nx, ldg::Point(-size, -size, -size), ldg::Point(size, size, size));
+    ldg::Mesh mesh{nx, nx, nx, ldg::Point(-size, -size, -size), ldg::Point(size, size, size), mpi};
     std::cout << "p = " << p << "\ntf = " << tf << "\nnx = " << nx << std::endl;
     ldg::Solver solver(p, dtSnaps, tf, L, mesh, opt);
     if (resume) solver.loadCheckpoint(resume);

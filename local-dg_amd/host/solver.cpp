@@ -27,6 +27,8 @@ Solver::Solver(int p, int dtSnaps_, double tf_, double L, const Mesh &mesh_, con
   check(ldg_create(&c, &h), "ldg_create");
   int nel = 0;
   ldg_sizes(h, &dofs_, &nel);
+  mesh.mpi.initDatatype((p + 1) * (p + 1));   // solver.cpp:58-59
+  mesh.mpi.initFaces(Mesh::N_FACES);
   // dt from the CFL rule, then rounded so that tf is hit exactly (solver.cpp:47-52)
   dt = ldg_time_step(h, opt.cfl);
   timesteps = (long)std::ceil(tf / dt);
@@ -80,10 +82,13 @@ void Solver::loadCheckpoint(const std::string &filename) {
   Checkpoint c;
   if (!readCheckpoint(filename, c)) throw std::runtime_error("could not read checkpoint " + filename);
   const bool same = c.p == order && c.nx == mesh.nx && c.ny == mesh.ny && c.nz == mesh.nz && c.timesteps == timesteps &&
+                    c.precision == opt.precision && c.L == L &&
                     c.dt == dt && c.Re == opt.Re && c.M0 == opt.M0 && c.V0 == opt.V0 && c.Pr == opt.Pr &&
                     c.lo[0] == mesh.botLeft.x && c.hi[0] == mesh.topRight.x && c.lo[1] == mesh.botLeft.y &&
                     c.hi[1] == mesh.topRight.y && c.lo[2] == mesh.botLeft.z && c.hi[2] == mesh.topRight.z;
-  if (!same) throw std::runtime_error("checkpoint " + filename + " is from a different run (order, mesh, physics or dt)");
+  if (!same)
+    throw std::runtime_error("checkpoint " + filename +
+                             " is from a different run (order, mesh, precision, L, physics or dt)");
   check(ldg_set_state(h, c.u.data()), "ldg_set_state");
   step = c.step;
   kes.assign(timesteps, 0.0);
@@ -91,19 +96,20 @@ void Solver::loadCheckpoint(const std::string &filename) {
 }
 
 void Solver::dgTimeStep() {
-  if (step == 0) kes.assign(timesteps, 0.0);
+  if (step == 0 || kes.size() != (std::size_t)timesteps) kes.resize(timesteps, 0.0);
   if (opt.verbose) std::cout << "Time stepping until tf = " << tf << std::endl;
   if (opt.snapshots) mkdir(opt.outputDir.c_str(), 0755);
   const auto start = std::chrono::steady_clock::now();
+  auto elapsed = [&] { return std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count(); };
+  double perStep = 0.0;   // measured seconds per step, to stop a call at the wall limit
   long &iStep = step;
   while (iStep < timesteps) {
-    const double elapsed = std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();
-    if (elapsed >= opt.wallLimitSeconds) {    // dump state and stop (solver.cpp:349-360)
+    if (elapsed() >= opt.wallLimitSeconds) {    // dump state and stop (solver.cpp:349-360)
+      if (!opt.checkpointFile.empty()) saveCheckpoint(opt.checkpointFile);
+      mkdir(opt.outputDir.c_str(), 0755);
       const std::string name = opt.outputDir + "/xyznavier.tmp";
       if (!initXYZVFile(name, 0, "u", 5) || !exportToXYZVFile(name, 0, coords, u()))
         throw std::runtime_error("could not write " + name);
-      if (!opt.checkpointFile.empty()) saveCheckpoint(opt.checkpointFile);
-      kes.resize(iStep);
       break;
     }
     if (opt.snapshots && iStep % dtSnaps == 0) {
@@ -112,18 +118,28 @@ void Solver::dgTimeStep() {
           !exportToXYZVFile(name, (int)(iStep / dtSnaps), coords, u()))
         throw std::runtime_error("could not write " + name);
     }
-    const long chunk = std::min<long>(timesteps - iStep, dtSnaps - iStep % dtSnaps);
+    // steps up to the next snapshot, but no more than fit before the wall
+    // limit at the measured rate, so the limit is checked between steps as in
+    // the reference (solver.cpp:349) without one call per step
+    long chunk = std::min<long>(timesteps - iStep, dtSnaps - iStep % dtSnaps);
+    if (perStep > 0) {
+      const double left = opt.wallLimitSeconds - elapsed();
+      chunk = std::max<long>(1, std::min<long>(chunk, (long)(left / perStep)));
+    } else {
+      chunk = 1;
+    }
     if (opt.verbose) std::cout << "time = " << iStep * dt << std::endl;
+    const double t0 = elapsed();
     check(ldg_step(h, dt, (int)chunk, kes.data() + iStep), "ldg_step");
+    perStep = (elapsed() - t0) / chunk;
     iStep += chunk;
   }
   if (iStep == timesteps && !opt.checkpointFile.empty()) saveCheckpoint(opt.checkpointFile);
-  // dissipation -(KE[i] - KE[i-2]) / (2 dt) (solver.cpp:437-440)
+  // dissipation -(KE[i] - KE[i-2]) / (2 dt) over the steps run (solver.cpp:437-440)
   keDiss.clear();
-  for (std::size_t i = 2; i < kes.size(); ++i) keDiss.push_back(-(kes[i] - kes[i - 2]) / (2 * dt));
-  const double elapsed = std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();
+  for (long i = 2; i < iStep; ++i) keDiss.push_back(-(kes[i] - kes[i - 2]) / (2 * dt));
   if (opt.verbose) {
-    std::cout << "Finished time stepping. Time elapsed = " << elapsed << std::endl;
+    std::cout << "Finished time stepping. Time elapsed = " << elapsed() << std::endl;
     std::cout << "Outputting kinetic energy dissipation for Navier-Stokes: " << std::endl;
     for (std::size_t i = 0; i < keDiss.size(); ++i) std::cout << keDiss[i] << (i + 1 < keDiss.size() ? ", " : "\n");
   }

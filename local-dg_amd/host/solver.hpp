@@ -25,7 +25,7 @@ struct SolverOptions {
   int precision = LDG_FP64;
   bool snapshots = true;          // output/xyzu.txt.<n> every dtSnaps steps (solver.cpp:366-381)
   std::string outputDir = "output";
-  double wallLimitSeconds = 19.5 * 60 * 60;   // solver.cpp:351
+  double wallLimitSeconds = 19.5 * 60 * 60;   // solver.cpp:351, checked before every step
   bool verbose = true;
   std::string checkpointFile;     // if set: binary checkpoint at the end of dgTimeStep and at the wall limit
 };
@@ -46,9 +46,11 @@ class Solver {
   void saveCheckpoint(const std::string &filename);
   void loadCheckpoint(const std::string &filename);
   long currentStep() const { return step; }
+  // replaces the wall-clock limit for later dgTimeStep calls
+  void setWallLimit(double seconds) { opt.wallLimitSeconds = seconds; }
 
-  // results and state
-  const std::vector<double> &kineticEnergy() const { return kes; }
+  // results and state: the KE history of the steps run so far
+  std::vector<double> kineticEnergy() const { return std::vector<double>(kes.begin(), kes.begin() + step); }
   const std::vector<double> &keDissipation() const { return keDiss; }
   double timeStep() const { return dt; }
   long timeSteps() const { return timesteps; }
@@ -65,7 +67,7 @@ class Solver {
   long timesteps, step = 0;
   int dtSnaps, order, dofs_;
   darray uHost, coords;
-  std::vector<double> kes, keDiss;
+  std::vector<double> kes, keDiss;   // kes has one entry per time step of the run
   void check(int rc, const char *what) const;
 };
 

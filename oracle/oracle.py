@@ -12,7 +12,9 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATHS = {"plain": os.path.join(_HERE, "_build", "libldg_oracle.so"),
-              "fma": os.path.join(_HERE, "_build", "libldg_oracle_fma.so")}
+              "fma": os.path.join(_HERE, "_build", "libldg_oracle_fma.so"),
+              "ld": os.path.join(_HERE, "_build", "libldg_oracle_ld.so"),
+              "native": os.path.join(_HERE, "_build", "libldg_oracle_native.so")}
 _libs = {}
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -23,12 +25,18 @@ def build():
 
 
 def lib(variant="plain"):
-    """The oracle library; variant "fma" is the same source built with fused
-    multiply-adds (used to measure the oracle's own round-off spread)."""
+    """The oracle library.  Variants of the same source: "plain" (FP64, the
+    oracle), "fma" (FP64 with fused multiply-adds: the oracle's own
+    round-off spread), "ld" (x87 long double throughout: the exact answer
+    the FP64 builds are measured against), "native" (FP64 for the building
+    host's vector units, bitwise equal to "plain"; fixture generation)."""
     if variant not in _libs:
         path = _LIB_PATHS[variant]
         if not os.path.exists(path):
-            build()
+            if variant == "native":
+                subprocess.run(["make", "-s", "-C", _HERE, "native"], check=True)
+            else:
+                build()
         L = ctypes.CDLL(path)
         L.orc_create.restype = ctypes.c_void_p
         L.orc_create.argtypes = [ctypes.c_int] * 4 + [_dp, _dp, ctypes.c_double, _dp]
@@ -53,6 +61,10 @@ def lib(variant="plain"):
         L.orc_fluxC.argtypes = [_dp, ctypes.c_double, _dp]
         L.orc_fluxV.argtypes = [_dp, _dp, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp]
         L.orc_roe.argtypes = [_dp, _dp, _dp, ctypes.c_double, _dp]
+        L.orc_real_size.restype = ctypes.c_int
+        L.orc_set_threads.argtypes = [ctypes.c_int]
+        L.orc_set_threads.restype = None
+        L.orc_max_threads.restype = ctypes.c_int
         _libs[variant] = L
     return _libs[variant]
 

@@ -54,13 +54,23 @@ def test_validate_compare_refuses_different_runs(tmp_path):
     raise AssertionError("runs of different length were compared")
 
 
-def test_profiled_figures_and_issue_bound():
-    # the committed profile carries the machine-readable lines bench.py reads
+def test_profiled_figures_and_issue_bound(tmp_path, monkeypatch):
+    """bench.py reads "key kernel precision value src=HASH" lines of the
+    rocprofv3 summary and drops any whose HASH is not that of the current
+    kernel sources."""
+    cur = bench.kernel_source_hash()
+    prof = tmp_path / "prof.md"
+    prof.write_text("    traffic rhs mixed 8635138049 src=%s\n"
+                    "    traffic rhs fp64 14481330586 src=000000000000\n"
+                    "    valu_insts rhs mixed 1083500000 src=%s\n" % (cur, cur))
+    monkeypatch.setattr(bench, "PROFILE", str(prof))
     t = bench.profiled_traffic("rhs", "mixed", 4, 64)
-    assert t is not None and t > bench.alg_bytes(163840000, True)[1]     # measured >= algorithmic
+    assert t == 8635138049 and t > bench.alg_bytes(163840000, True)[1]     # measured >= algorithmic
+    assert bench.profiled_traffic("rhs", "fp64", 4, 64) is None             # measured on other kernel code
     assert bench.profiled_traffic("rhs", "mixed", 4, 32) is None            # other workloads: not measured
     ib = bench.issue_bound("rhs", "mixed", 4, 64, 2.0)
     v = ib["valu_insts_per_launch"]
+    assert v == 1083500000
     assert abs(ib["busy_frac"] - v * 4 / (1024 * 2.4e9 * 2.0e-3)) < 1e-12
     assert bench.issue_bound("rhs", "mixed", 4, 32, 2.0) is None
 

@@ -1,0 +1,70 @@
+"""Taylor-Green on 32^3 elements, p = 4 (20.48 M DOF), FP64 on the GPU for
+1000 RK4 steps at the reference time step, against the oracle's histories
+in tests/golden/c2_tgv32_p4_history.npz (written by
+tests/golden/make_c2_history.py):
+  - the reference kinetic energy (Solver::computeKE, solver.cpp:1007-1034)
+    after every step (kes, solver.cpp:437), within 1e-12 relative;
+  - the Gauss-point kinetic energy (1e-12) and enstrophy (1e-10) every 10
+    steps (no reference counterpart);
+  - the final state on 32 sampled elements, within 1e-12 relative L2 per
+    conserved variable (rho w: within 4 x the FP64 oracle's own rounding
+    spread at this horizon, measured by the fixture script's sibling test);
+  - the lagged dissipation -dEk/dt (solver.cpp:437-440) within 1e-7
+    relative where it is larger than 1e-4 of its maximum."""
+import os
+
+import numpy as np
+import pytest
+
+import ldg_amd
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+FIXTURE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "c2_tgv32_p4_history.npz")
+
+
+def dissipation(ke, dt):
+    """-(KE[i] - KE[i-2]) / (2 dt), the reference's lagged central difference."""
+    return -(ke[2:] - ke[:-2]) / (2 * dt)
+
+
+@pytest.mark.skipif(not os.path.exists(FIXTURE), reason="fixture not generated yet (tests/golden/make_c2_history.py)")
+def test_c2_histories_match_oracle():
+    f = np.load(FIXTURE)
+    p, nx, nsteps, every = int(f["p"]), int(f["nx"]), int(f["nsteps"]), 10
+    print(f"C2: {nx}^3 p={p}, {nsteps} FP64 steps", flush=True)
+    u0 = O.Oracle(p, nx).initial_condition()
+    g = ldg_amd.LDG(p, nx)
+    dt = float(f["dt"])
+    assert g.time_step(float(f["cfl"])) == dt
+    g.set_state(u0)
+    d = g.diagnostics()
+    ke_g, ens = [d["ke"]], [d["enstrophy"]]
+    ke = []
+    for _ in range(nsteps // every):
+        ke.append(g.step(dt, every, record_ke=True))
+        d = g.diagnostics()
+        ke_g.append(d["ke"])
+        ens.append(d["enstrophy"])
+    ke = np.concatenate(ke)
+    eke = np.max(np.abs(ke - f["ke"]) / f["ke"])
+    ekg = np.max(np.abs(np.array(ke_g) - f["ke_gauss"]) / f["ke_gauss"])
+    een = np.max(np.abs(np.array(ens) - f["enstrophy"]) / f["enstrophy"])
+    diss, diss_ref = dissipation(ke, dt), dissipation(f["ke"], dt)
+    big = np.abs(diss_ref) > 1e-4 * np.abs(diss_ref).max()
+    edi = np.max(np.abs(diss - diss_ref)[big] / np.abs(diss_ref[big]))
+    u = g.get_state()
+    samp = u[f["sample_elems"]]
+    ref = f["sample_state"]
+    es = np.sqrt(np.sum((samp - ref) ** 2, axis=(0, 2))) / np.sqrt(np.sum(ref ** 2, axis=(0, 2)))
+    en = np.abs(np.sqrt(np.sum(u ** 2, axis=(0, 2))) - f["final_norms"]) / f["final_norms"]
+    print(f"C2 max relative difference: KE {eke:.2e}, Gauss KE {ekg:.2e}, enstrophy {een:.2e}, "
+          f"-dEk/dt {edi:.2e}; sampled state per variable {es}; norms {en}", flush=True)
+    assert eke <= 1e-12
+    assert ekg <= 1e-12
+    assert een <= 1e-10
+    assert edi <= 1e-7
+    assert np.all(es[[0, 1, 2, 4]] <= 1e-12), es
+    assert es[3] <= 1e-10, es
+    g.close()

@@ -278,3 +278,44 @@ def test_corrected_run_is_deterministic():
         g.close()
     np.testing.assert_array_equal(out[0][0], out[1][0])
     np.testing.assert_array_equal(out[0][1], out[1][1])
+
+
+def test_adam_matches_numpy_adam():
+    """Five full-batch Adam steps of ldg_closure_train against Algorithm 1
+    of Kingma & Ba (2015) in float64 numpy: m_t = b1 m + (1-b1) g,
+    v_t = b2 v + (1-b2) g^2, theta -= lr (m_t/(1-b1^t)) / (sqrt(v_t/(1-b2^t)) + eps),
+    with g the GPU gradient at the previous parameters (ldg_closure_gradient
+    on the same normalised batch).  A missing or wrong bias correction moves
+    the first step by a factor of about 3 and fails the check."""
+    n, steps, lr = 256, 5, 1e-3
+    b1, b2, eps = 0.9, 0.999, 1e-8
+    rng = np.random.default_rng(21)
+    c = C.Closure(2, 256, n, seed=31)
+    c.add_samples(rng.standard_normal((n, c.nin)).astype(np.float32),
+                  rng.standard_normal((n, c.nout)).astype(np.float32))
+    c.fit_normalization()
+    Xn, Tn = c.get_samples()
+    theta0, norm = c.get()
+    probe = C.Closure(2, 256, 0, seed=0)
+    theta = theta0.astype(np.float64)
+    m = np.zeros_like(theta)
+    v = np.zeros_like(theta)
+    for t in range(1, steps + 1):
+        probe.set(theta.astype(np.float32), norm)
+        g, _ = probe.gradient(Xn, Tn)
+        g = g.astype(np.float64)
+        m = b1 * m + (1 - b1) * g
+        v = b2 * v + (1 - b2) * g * g
+        theta = theta - lr * (m / (1 - b1 ** t)) / (np.sqrt(v / (1 - b2 ** t)) + eps)
+    c.train(steps, batch=n, lr=lr, val_fraction=0.0, seed=3)
+    theta_gpu = c.get()[0].astype(np.float64)
+    d_ref, d_gpu = theta - theta0, theta_gpu - theta0
+    err = np.abs(d_gpu - d_ref)
+    print(f"Adam: max |step| {np.abs(d_ref).max():.3e}, max |gpu - numpy| {err.max():.3e}, "
+          f"median {np.median(err):.3e}", flush=True)
+    # every parameter moves by about lr per step early in Adam
+    assert np.median(np.abs(d_ref)) > 0.5 * lr * steps
+    assert np.median(err) <= 1e-3 * np.median(np.abs(d_ref))
+    assert np.quantile(err, 0.999) <= 5e-2 * np.abs(d_ref).max()
+    probe.close()
+    c.close()

@@ -1,14 +1,16 @@
 """GPU parity of libldg.so (through its C ABI) against the CPU oracle.
 
-Bar: every conserved variable within relative L2 TOL = 1e-12 of the FP64
-oracle, except where the oracle itself is not reproducible to that level:
-the same oracle source built with fused multiply-adds (an equally valid FP64
-rounding) differs from the plain build by `spread`, and a variable's
-tolerance is then max(TOL, FLOOR_FACTOR * spread).  On the 100-step
-8^3, p=2 run this matters only for rho w, which starts at zero (spread
-~2.5e-12); on single right-hand sides it matters for the momentum, where the
-O(1/M0^2) background pressure cancels.  Bitwise where the computation is
-identical by construction (periodic shifts)."""
+Two oracles built from the same source (oracle/ldg_oracle.c): the FP64
+build, which is the reference's algorithm in the reference's precision, and
+the x87 long double build ("ld"), which carries every operation 11 bits
+further and is the exact answer to better than 1e-15 here.  The bar of
+1e-12 relative L2 per conserved variable is applied against the exact
+answer.  The FP64 oracle is itself up to 2.6e-12 (rho w after 100 C1 steps)
+and 4e-10 (a single p = 4 right-hand side) away from it: the O(1/M0^2)
+background pressure cancels in its dense operators and fluxes.  The GPU
+path evaluates fluxes on fluctuations about the background and is tested to
+be at least as close to the exact answer as the FP64 oracle is.  Bitwise
+where the computation is identical by construction (periodic shifts)."""
 import numpy as np
 import pytest
 
@@ -51,31 +53,45 @@ CASES = [(1, 3, 3, 3), (2, 4, 4, 4), (3, 3, 4, 5), (4, 4, 4, 4), (5, 3, 3, 3), (
 @pytest.mark.parametrize("p,nx,ny,nz", CASES)
 @pytest.mark.parametrize("state", ["tgv", "perturbed"])
 def test_rhs_matches_oracle(p, nx, ny, nz, state):
+    """One right-hand side and its auxiliary gradient.  Against the exact
+    (long double) answer: k within max(1e-12, the FP64 oracle's own error);
+    against the FP64 oracle: within max(1e-12, 4 x the spread between its
+    plain and fused-multiply-add builds)."""
     print(f"rhs parity p={p} mesh={nx}x{ny}x{nz} state={state}", flush=True)
     o = O.Oracle(p, nx, ny, nz)
     of = O.Oracle(p, nx, ny, nz, variant="fma")
+    ol = O.Oracle(p, nx, ny, nz, variant="ld")
     g = ldg_amd.LDG(p, nx, ny, nz)
     u = o.initial_condition() if state == "tgv" else perturbed(o, 100 + p)
     k_ref, du_ref = o.rhs(u, want_du=True)
     k_fma, du_fma = of.rhs(u, want_du=True)
+    k_ex, du_ex = ol.rhs(u, want_du=True)
     k, du = g.rhs(u, want_du=True)
+    e_gpu, e_orc = rel_l2(k, k_ex), rel_l2(k_ref, k_ex)
+    print("  k rel L2 vs exact: GPU", e_gpu, "FP64 oracle", e_orc, flush=True)
+    assert np.all(e_gpu <= np.maximum(TOL, e_orc)), (e_gpu, e_orc)
     ek = rel_l2(k, k_ref)
     tol = np.maximum(TOL, FLOOR_FACTOR * rel_l2(k_fma, k_ref))
-    print("  k rel L2", ek, "tol", tol, flush=True)
     assert np.all(ek <= tol), (ek, tol)
     for l in range(3):
         # rho w and its gradients vanish identically for the TGV state
-        nz_mask = np.sqrt(np.sum(du_ref[l] ** 2, axis=(0, 2))) > 1e-10
+        nz_mask = np.sqrt(np.sum(du_ex[l] ** 2, axis=(0, 2))) > 1e-10
+        e = rel_l2(du[l], du_ex[l])[nz_mask]
+        eo = rel_l2(du_ref[l], du_ex[l])[nz_mask]
+        assert np.all(e <= np.maximum(TOL, eo)), (l, e, eo)
         e = rel_l2(du[l], du_ref[l])[nz_mask]
         tl = np.maximum(TOL, FLOOR_FACTOR * rel_l2(du_fma[l], du_ref[l]))[nz_mask]
         assert np.all(e <= tl), (l, e, tl)
 
 
-def test_c1_hundred_steps_match_oracle():
-    """Taylor-Green, 8^3 elements, p=2, 100 RK4 steps at the reference
-    time step (CFL 0.01): every conserved variable within relative L2 1e-12
-    (rho w: within FLOOR_FACTOR x the oracle's own rounding spread), and
-    the kinetic-energy history within 1e-12 relative."""
+def test_c1_hundred_steps_match_exact_answer():
+    """Taylor-Green, 8^3 elements, p=2, 100 RK4 steps at the reference time
+    step (CFL 0.01): every conserved variable within relative L2 1e-12 of the
+    exact (long double) answer, rho w included, and the kinetic-energy
+    history within 1e-12 relative.  The FP64 oracle's own distance from the
+    exact answer is printed beside the GPU's (rho w: 2.6e-12), and the GPU
+    must also agree with the FP64 oracle to within 4 x that oracle's own
+    rounding spread."""
     p, nx, nsteps = 2, 8, 100
     o = O.Oracle(p, nx)
     g = ldg_amd.LDG(p, nx)
@@ -86,12 +102,14 @@ def test_c1_hundred_steps_match_oracle():
     ke = g.step(dt, nsteps)
     u = g.get_state()
     u_ref, ke_ref = o.step(u0, dt, nsteps)
+    u_ex, ke_ex = O.Oracle(p, nx, variant="ld").step(u0, dt, nsteps)
     u_fma, _ = O.Oracle(p, nx, variant="fma").step(u0, dt, nsteps, record_ke=False)
+    e_gpu, e_orc = rel_l2(u, u_ex), rel_l2(u_ref, u_ex)
+    print("C1 per-variable relative L2 after 100 steps vs exact: GPU", e_gpu, "FP64 oracle", e_orc, flush=True)
+    assert np.all(e_gpu <= TOL), e_gpu
+    assert np.max(np.abs(ke - ke_ex) / np.abs(ke_ex)) <= TOL
     e = rel_l2(u, u_ref)
-    spread = rel_l2(u_fma, u_ref)
-    tol = np.maximum(TOL, FLOOR_FACTOR * spread)
-    print("C1 per-variable relative L2 after 100 steps:", e, "oracle spread:", spread, flush=True)
-    assert np.all(e[[0, 1, 2, 4]] <= TOL), e
+    tol = np.maximum(TOL, FLOOR_FACTOR * rel_l2(u_fma, u_ref))
     assert np.all(e <= tol), (e, tol)
     assert np.max(np.abs(ke - ke_ref) / np.abs(ke_ref)) <= TOL
 
@@ -288,3 +306,61 @@ def test_cpp_driver_wall_limit_dumps_and_stops_cleanly(tmp_path):
     assert end["step"] == end["timesteps"]
     assert np.array_equal(end["u"], u_all)
     assert np.array_equal(end["ke"], ke_all)
+
+
+def test_cpp_driver_wall_limit_without_snapshots_writes_checkpoint(tmp_path):
+    """-nosnap with a passed wall limit: the output directory is created for
+    the state dump, the checkpoint is written, and the run returns 0."""
+    import os
+    import subprocess
+    from ldg_amd import io
+    exe = os.path.join(os.path.dirname(ldg_amd.LIB_PATH), "ldg_driver")
+    out = tmp_path / "never_made"
+    r = subprocess.run([exe, "-p", "2", "-nx", "4", "-tf", "0.02", "-nosnap", "-quiet", "-out", str(out),
+                        "-wallLimit", "0", "-checkpoint", str(tmp_path / "w.ckpt")],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert io.load_checkpoint(tmp_path / "w.ckpt")["step"] == 0
+    assert (out / "xyznavier.tmp.0").stat().st_size > 0
+
+
+def test_cpp_host_check_on_the_gpu(tmp_path):
+    """tests/host_solver_check.cpp (the reference's driver block, a wall-limit
+    stop continued and resumed, checkpoints refused across precisions)
+    against the real libldg.so: the continued and the resumed run end with
+    the same KE history bitwise."""
+    import os
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    host = os.path.join(root, "local-dg_amd", "host")
+    libdir = os.path.dirname(ldg_amd.LIB_PATH)
+    exe = tmp_path / "host_solver_check"
+    subprocess.run(["g++", "-std=c++17", "-O1", "-Wall", "-I" + host, "-I" + os.path.join(root, "include"),
+                    os.path.join(root, "tests", "host_solver_check.cpp")] +
+                   [os.path.join(host, f) for f in ("mesh.cpp", "solver.cpp", "io.cpp")] +
+                   ["-L" + libdir, "-lldg", "-Wl,-rpath," + libdir, "-o", str(exe)], check=True, capture_output=True,
+                   text=True)
+    r = subprocess.run([str(exe), str(tmp_path)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert r.stdout.strip().endswith("ok")
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_long_calls_record_ke_in_chunks(graph):
+    """The device KE history holds 1024 steps; a longer call copies it out in
+    chunks.  2,101 steps in one call and in calls of 700, 1 and 1,400 steps
+    end with the same state and KE history, bitwise (eager and graph
+    replay)."""
+    print(f"KE history chunks, graph={graph}", flush=True)
+    runs = []
+    for split in ([2101], [700, 1, 1400]):
+        g = ldg_amd.LDG(1, 3)
+        g.initial_condition()
+        g.set_graph(graph)
+        dt = g.time_step(0.01)
+        ke = np.concatenate([g.step(dt, n, record_ke=True) for n in split])
+        runs.append((g.get_state(), ke))
+        g.close()
+    assert runs[0][1].shape == (2101,)
+    assert np.array_equal(runs[0][0], runs[1][0])
+    assert np.array_equal(runs[0][1], runs[1][1])
