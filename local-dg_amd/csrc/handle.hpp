@@ -59,6 +59,7 @@ struct Group {
   double *u = nullptr, *acc = nullptr, *k = nullptr;
   void *ua = nullptr, *ub = nullptr;  // stage inputs: FP64 state, or FP32 fluctuations in the mixed path
   void *q = nullptr;
+  void *ff = nullptr;     // face fluxes written by the gradient kernel, lifted by the rhs kernel
   double *ke_part = nullptr, *diag_part = nullptr, *d_scale = nullptr;
   void *ops = nullptr;
   Tiling rhs[kMaxCand], grad[kMaxCand];
@@ -74,7 +75,13 @@ struct ldg_handle {
   bool mixed = false;
   std::vector<ldg_host::Inst> inst;
   std::vector<ldg_host::Group> groups;
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr;     // the handle's stream: every call is ordered on it
+  // ensembles of several orders: one stream per group (default priority),
+  // forked from and joined back to `stream` in every RK4 step, so that the
+  // groups' launches overlap (graph capture records them as parallel branches)
+  std::vector<hipStream_t> gstreams;
+  hipEvent_t ev_fork = nullptr;
+  std::vector<hipEvent_t> ev_join;
   double *ke_hist = nullptr, *scal = nullptr;
   int *d_step = nullptr;
   size_t ke_hist_cap = 0;          // kKeHistSteps x instances, allocated at create

@@ -138,13 +138,13 @@ void launch_grad_e(ldg_handle *h, Group &g, const Tiling &t, StageIn U) {
   const dim3 grid(8 * la.per_xcd), block(NT);
   if (h->mixed && U.fluct)
     hipLaunchKernelGGL((ldg_grad_kernel<N, E, float, float>), grid, block, 0, h->stream, (const float *)U.p,
-                       (float *)g.q, la, op);
+                       (float *)g.q, (float *)g.ff, la, op);
   else if (h->mixed)
     hipLaunchKernelGGL((ldg_grad_kernel<N, E, float, double>), grid, block, 0, h->stream, (const double *)U.p,
-                       (float *)g.q, la, op);
+                       (float *)g.q, (float *)g.ff, la, op);
   else
     hipLaunchKernelGGL((ldg_grad_kernel<N, E, double, double>), grid, block, 0, h->stream, (const double *)U.p,
-                       (double *)g.q, la, op);
+                       (double *)g.q, (double *)g.ff, la, op);
 }
 
 template <int N, int E>
@@ -154,13 +154,13 @@ void launch_rhs_e(ldg_handle *h, Group &g, const Tiling &t, StageIn U, const Sta
   const dim3 grid(8 * la.per_xcd), block(NT);
   if (h->mixed && U.fluct)
     hipLaunchKernelGGL((ldg_rhs_kernel<N, E, float, float>), grid, block, 0, h->stream, (const float *)U.p,
-                       (const float *)g.q, la, op, st);
+                       (const float *)g.q, (const float *)g.ff, la, op, st);
   else if (h->mixed)
     hipLaunchKernelGGL((ldg_rhs_kernel<N, E, float, double>), grid, block, 0, h->stream, (const double *)U.p,
-                       (const float *)g.q, la, op, st);
+                       (const float *)g.q, (const float *)g.ff, la, op, st);
   else
     hipLaunchKernelGGL((ldg_rhs_kernel<N, E, double, double>), grid, block, 0, h->stream, (const double *)U.p,
-                       (const double *)g.q, la, op, st);
+                       (const double *)g.q, (const double *)g.ff, la, op, st);
 }
 
 template <int N>
@@ -228,6 +228,37 @@ int upload_ops(ldg_handle *h, Group &g) {
   return fail(h, LDG_EINVAL, "unsupported order");
 }
 
+// Instance table entry of instance `in` for a tiling of E elements per
+// workgroup, with its first element and first logical workgroup.
+int inst_desc(const ldg_config &cfg, const Inst &in, int E, long elem_off, int block_off, InstDesc &d) {
+  std::memset(&d, 0, sizeof(d));
+  d.nx = in.nx; d.ny = in.ny; d.nz = in.nz;
+  d.nelem = in.K;
+  d.elem_off = elem_off;
+  // blocks of E elements along x-rows, visited in bricks of about 8 x 8 x 4
+  // elements (whole bricks; blocks outside the mesh are padding)
+  d.nbx = (in.nx + E - 1) / E;
+  d.tb[0] = std::min(std::max(1, 8 / E), d.nbx);
+  d.tb[1] = std::min(8, in.ny);
+  d.tb[2] = std::min(4, in.nz);
+  d.nbrick[0] = (d.nbx + d.tb[0] - 1) / d.tb[0];
+  d.nbrick[1] = (in.ny + d.tb[1] - 1) / d.tb[1];
+  d.nbrick[2] = (in.nz + d.tb[2] - 1) / d.tb[2];
+  const long nb = (long)d.nbrick[0] * d.nbrick[1] * d.nbrick[2] * d.tb[0] * d.tb[1] * d.tb[2];
+  if (nb > (1L << 30)) return LDG_EINVAL;
+  d.nblocks = (int)nb;
+  d.block_off = block_off;
+  for (int l = 0; l < 3; ++l) d.ialpha[l] = 2.0 / in.dx[l];
+  d.dt = 0.0;
+  d.ph = phys_args(cfg, in);
+  d.div_bs = make_fastdiv(d.tb[0] * d.tb[1] * d.tb[2]);
+  d.div_nb0 = make_fastdiv(d.nbrick[0]);
+  d.div_nb1 = make_fastdiv(d.nbrick[1]);
+  d.div_tb0 = make_fastdiv(d.tb[0]);
+  d.div_tb1 = make_fastdiv(d.tb[1]);
+  return LDG_OK;
+}
+
 int build_tiling(ldg_handle *h, Group &g, Tiling &t, int E) {
   t.E = E;
   t.h_inst.clear();
@@ -237,31 +268,8 @@ int build_tiling(ldg_handle *h, Group &g, Tiling &t, int E) {
   for (int idx : g.members) {
     const Inst &in = h->inst[idx];
     InstDesc d;
-    std::memset(&d, 0, sizeof(d));
-    d.nx = in.nx; d.ny = in.ny; d.nz = in.nz;
-    d.nelem = in.K;
-    d.elem_off = off;
-    // blocks of E elements along x-rows, visited in bricks of about 8 x 8 x 4
-    // elements (whole bricks; blocks outside the mesh are padding)
-    d.nbx = (in.nx + E - 1) / E;
-    d.tb[0] = std::min(std::max(1, 8 / E), d.nbx);
-    d.tb[1] = std::min(8, in.ny);
-    d.tb[2] = std::min(4, in.nz);
-    d.nbrick[0] = (d.nbx + d.tb[0] - 1) / d.tb[0];
-    d.nbrick[1] = (in.ny + d.tb[1] - 1) / d.tb[1];
-    d.nbrick[2] = (in.nz + d.tb[2] - 1) / d.tb[2];
-    const long nb = (long)d.nbrick[0] * d.nbrick[1] * d.nbrick[2] * d.tb[0] * d.tb[1] * d.tb[2];
-    if (nb > (1L << 30)) return fail(h, LDG_EINVAL, "mesh too large for the block numbering");
-    d.nblocks = (int)nb;
-    d.block_off = boff;
-    for (int l = 0; l < 3; ++l) d.ialpha[l] = 2.0 / in.dx[l];
-    d.dt = 0.0;
-    d.ph = phys_args(h->cfg, in);
-    d.div_bs = make_fastdiv(d.tb[0] * d.tb[1] * d.tb[2]);
-    d.div_nb0 = make_fastdiv(d.nbrick[0]);
-    d.div_nb1 = make_fastdiv(d.nbrick[1]);
-    d.div_tb0 = make_fastdiv(d.tb[0]);
-    d.div_tb1 = make_fastdiv(d.tb[1]);
+    if (inst_desc(h->cfg, in, E, off, boff, d) != LDG_OK)
+      return fail(h, LDG_EINVAL, "mesh too large for the block numbering");
     for (int b = 0; b < d.nblocks; ++b) blk.push_back(in.slot);
     off += in.K;
     boff += d.nblocks;
@@ -340,6 +348,18 @@ int create_impl(const ldg_config *cfg, const ldg_instance *insts, int n, ldg_han
     h->err = "hipStreamCreate failed";
     return bad(LDG_EDEVICE);
   }
+  if (h->groups.size() > 1) {
+    h->gstreams.assign(h->groups.size(), nullptr);
+    h->ev_join.assign(h->groups.size(), nullptr);
+    bool ok = hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming) == hipSuccess;
+    for (size_t gi = 0; ok && gi < h->groups.size(); ++gi)
+      ok = hipStreamCreateWithFlags(&h->gstreams[gi], hipStreamNonBlocking) == hipSuccess &&
+           hipEventCreateWithFlags(&h->ev_join[gi], hipEventDisableTiming) == hipSuccess;
+    if (!ok) {
+      h->err = "hipStreamCreate failed (group streams)";
+      return bad(LDG_EDEVICE);
+    }
+  }
   for (Group &g : h->groups) {
     g.nU = (size_t)g.n * g.n * g.n * NSTATE * g.K;
     g.basis = make_basis(g.n - 1);
@@ -354,6 +374,11 @@ int create_impl(const ldg_config *cfg, const ldg_instance *insts, int n, ldg_han
     }
     if (hipMalloc(&g.q, g.nU * 3 * (h->mixed ? sizeof(float) : sizeof(double))) != hipSuccess) {
       h->err = "device allocation failed (gradient)";
+      return bad(LDG_ENOMEM);
+    }
+    // face fluxes: per element 3 directions x 2 sides x 5 states x n^2 points
+    if (hipMalloc(&g.ff, (size_t)g.K * 30 * g.n * g.n * (h->mixed ? sizeof(float) : sizeof(double))) != hipSuccess) {
+      h->err = "device allocation failed (face fluxes)";
       return bad(LDG_ENOMEM);
     }
     int maxblk = 0;
@@ -409,8 +434,8 @@ int set_dt(ldg_handle *h, const double *dt) {
 }
 
 // One RK4 step of every group (eager launches).  With ke, each group's
-// kinetic energies of u^{n+1} go to row *d_step of the KE history.
-// One RK4 step of every group.  In the mixed path without a closure, a step
+// kinetic energies of u^{n+1} go to row *d_step of the KE history.  In the
+// mixed path without a closure, a step
 // that follows another one in the same call (in32) reads its stage-1 input
 // as the FP32 fluctuations the previous step's stage 4 left in ub (out32);
 // ub is free then (stage 4 reads ua) and is only rewritten by stage 2.  The
@@ -419,8 +444,24 @@ int set_dt(ldg_handle *h, const double *dt) {
 int rk4_step(ldg_handle *h, bool ke, bool in32, bool out32) {
   static const double b[4] = {1 / 6.0, 1 / 3.0, 1 / 3.0, 1 / 6.0};
   static const double a[3] = {0.5, 0.5, 1.0};
+  // with several order groups each runs on its own stream between a fork
+  // from and a join back to the handle's stream; the launch helpers use
+  // h->stream, which points at the group's stream meanwhile
+  const bool fork = !h->gstreams.empty();
+  hipStream_t const main = h->stream;
+  struct Restore {
+    ldg_handle *h;
+    hipStream_t s;
+    ~Restore() { h->stream = s; }
+  } restore{h, main};
+  if (fork) HIPCHECK(h, hipEventRecord(h->ev_fork, main));
   int first = 0;
-  for (Group &g : h->groups) {
+  for (size_t gi = 0; gi < h->groups.size(); ++gi) {
+    Group &g = h->groups[gi];
+    if (fork) {
+      h->stream = h->gstreams[gi];
+      HIPCHECK(h, hipStreamWaitEvent(h->stream, h->ev_fork, 0));
+    }
     const bool carry = h->mixed && g.cl.net == nullptr;
     StageIn U = carry && in32 ? StageIn{g.ub, true} : state_in(g.u);
     void *bufs[2] = {g.ua, g.ub};
@@ -452,9 +493,15 @@ int rk4_step(ldg_handle *h, bool ke, bool in32, bool out32) {
                          (const double *)g.d_scale, h->ke_hist + first, (const int *)h->d_step);
       HIPCHECK(h, hipGetLastError());
     }
+    if (fork) {
+      HIPCHECK(h, hipEventRecord(h->ev_join[gi], h->stream));
+      h->stream = main;
+    }
     std::swap(g.u, g.acc);
     first += (int)g.members.size();
   }
+  if (fork)
+    for (hipEvent_t ev : h->ev_join) HIPCHECK(h, hipStreamWaitEvent(main, ev, 0));
   if (ke) {
     hipLaunchKernelGGL(ldg_advance_kernel, dim3(1), dim3(1), 0, h->stream, h->d_step);
     HIPCHECK(h, hipGetLastError());
@@ -673,6 +720,7 @@ void ldg_destroy(ldg_handle *h) {
     if (g.ua) (void)hipFree(g.ua);
     if (g.ub) (void)hipFree(g.ub);
     if (g.q) (void)hipFree(g.q);
+    if (g.ff) (void)hipFree(g.ff);
     if (g.ops) (void)hipFree(g.ops);
     for (int c = 0; c < kMaxCand; ++c) { free_tiling(g.rhs[c]); free_tiling(g.grad[c]); }
     closure_release(g);
@@ -680,6 +728,9 @@ void ldg_destroy(ldg_handle *h) {
   if (h->ke_hist) (void)hipFree(h->ke_hist);
   if (h->scal) (void)hipFree(h->scal);
   if (h->d_step) (void)hipFree(h->d_step);
+  for (hipStream_t gs : h->gstreams) if (gs) (void)hipStreamDestroy(gs);
+  for (hipEvent_t ev : h->ev_join) if (ev) (void)hipEventDestroy(ev);
+  if (h->ev_fork) (void)hipEventDestroy(h->ev_fork);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
 }

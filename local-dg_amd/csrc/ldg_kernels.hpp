@@ -268,12 +268,36 @@ __device__ inline void volume_flux(const T w[5], const T *g, const Bg<T> &b, T F
   }
 }
 
-// Face flux for element K on the face with outward normal sgn*e_D:
-// Roe(uN, uK, n) (numericalFluxC, solver.cpp:1194-1374) plus the one-sided
-// viscous flux -Fv_D(uK, q) sgn (numericalFluxV, solver.cpp:891-921; q is
-// the neighbour's gradient on positive faces, the element's own otherwise).
+// Fluxes of one face, for both elements that share it.  The face has the
+// element K on its - side and the element N on its + side (normal +e_D out
+// of K); wK, wN are the two traces and q the gradient trace chosen by the
+// one-sided switch, which is N's own on this face for both sides (the
+// switch takes the + side element's gradient: the neighbour's for K, whose
+// normal sum is positive, and N's own for N).  Returns K's flux
+//   outK = Roe(uN, uK, +e_D) - Fv_D(uK, q)
+// (numericalFluxC, solver.cpp:1194-1374; numericalFluxV, solver.cpp:891-921)
+// and N's flux on the same face, whose normal is -e_D,
+//   outN = Roe(uK, uN, -e_D) + Fv_D(uN, q) = -Roe(uN, uK, +e_D) + Fv_D(uN, q):
+// the Roe flux is evaluated once and enters the two sides with opposite
+// signs (F*(a, b, n) = -F*(b, a, -n): the Roe average is symmetric in the
+// two states and |A(-n)| = |A(n)|), while the viscous flux is evaluated on
+// each side's own trace, as the reference does.
 template <typename T, int D>
-__device__ inline void face_flux(const T wK[5], const T wN[5], const T *q, T sgn, const Bg<T> &b, T out[5]) {
+__device__ inline void visc_face(const T w[5], T ir, const T v[3], const T *q, const Bg<T> &b, T fv[4]) {
+  T gv[3][3], ge[3];
+  prim_grads<T>(ir, b.E0 + w[4], v, q, gv, ge);
+  const T div = gv[0][0] + gv[1][1] + gv[2][2];
+  T tau[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) tau[i] = b.mu * (gv[i][D] + gv[D][i]) - (i == D ? b.mu * T(2.0 / 3.0) * div : T(0));
+#pragma unroll
+  for (int i = 0; i < 3; ++i) fv[i] = tau[i];
+  fv[3] = v[0] * tau[0] + v[1] * tau[1] + v[2] * tau[2] + b.kappa * ge[D];
+}
+
+template <typename T, int D>
+__device__ inline void face_flux_pair(const T wK[5], const T wN[5], const T *q, const Bg<T> &b, T outK[5],
+                                      T outN[5]) {
   const T rN = b.rho0 + wN[0], rK = b.rho0 + wK[0];
   const T irN = rcp(rN), irK = rcp(rK);
   const T vN[3] = {wN[1] * irN, wN[2] * irN, wN[3] * irN};
@@ -282,7 +306,7 @@ __device__ inline void face_flux(const T wK[5], const T wN[5], const T *q, T sgn
   const T pK = b.gm1 * (wK[4] - T(0.5) * (wK[1] * vK[0] + wK[2] * vK[1] + wK[3] * vK[2]));
   const T hN = b.H0 + wN[4] + pN, hK = b.H0 + wK[4] + pK;                                   // E + p
   const T HN = hN * irN, HK = hK * irK;
-  const T qN = sgn * vN[D], qK = sgn * vK[D];
+  const T qN = vN[D], qK = vK[D];
   const T s = sqrt_(rN * irK);
   const T iden = rcp(s + T(1));
   T vt[3];
@@ -290,7 +314,7 @@ __device__ inline void face_flux(const T wK[5], const T wN[5], const T *q, T sgn
   for (int i = 0; i < 3; ++i) vt[i] = (s * vN[i] + vK[i]) * iden;
   const T ht = (s * HN + HK) * iden;
   const T q2 = vt[0] * vt[0] + vt[1] * vt[1] + vt[2] * vt[2];
-  const T qt = sgn * vt[D];
+  const T qt = vt[D];
   const T c2 = b.gm1 * (ht - T(0.5) * q2);
   const T c = sqrt_(c2), ic = rcp(c);
   const T l0 = fabs(qt), lp = fabs(qt + c), lm = fabs(qt - c);
@@ -298,29 +322,29 @@ __device__ inline void face_flux(const T wK[5], const T wN[5], const T *q, T sgn
 #pragma unroll
   for (int k = 0; k < 5; ++k) dd[k] = wN[k] - wK[k];
   const T dp = b.gm1 * (T(0.5) * q2 * dd[0] - (vt[0] * dd[1] + vt[1] * dd[2] + vt[2] * dd[3]) + dd[4]);
-  const T dV = sgn * dd[1 + D] - qt * dd[0];
+  const T dV = dd[1 + D] - qt * dd[0];
   const T s1 = T(0.5) * (lp + lm) - l0, s2 = T(0.5) * (lp - lm);
   const T A = s1 * dp * ic * ic + s2 * dV * ic;
   const T B = s2 * dp * ic + s1 * dV;
-  out[0] = T(0.5) * (rN * qN + rK * qK) - T(0.5) * (l0 * dd[0] + A);
+  T R[5];
+  R[0] = T(0.5) * (rN * qN + rK * qK) - T(0.5) * (l0 * dd[0] + A);
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
-    const T pn = (i == D) ? sgn * (pN + pK) : T(0);
-    const T bn = (i == D) ? sgn * B : T(0);
-    out[1 + i] = T(0.5) * (wN[1 + i] * qN + wK[1 + i] * qK + pn) - T(0.5) * (l0 * dd[1 + i] + A * vt[i] + bn);
+    const T pn = (i == D) ? (pN + pK) : T(0);
+    const T bn = (i == D) ? B : T(0);
+    R[1 + i] = T(0.5) * (wN[1 + i] * qN + wK[1 + i] * qK + pn) - T(0.5) * (l0 * dd[1 + i] + A * vt[i] + bn);
   }
-  out[4] = T(0.5) * (hN * qN + hK * qK) - T(0.5) * (l0 * dd[4] + A * ht + B * qt);
-
-  // viscous part: only Fv_D is needed since n = sgn e_D
-  T gv[3][3], ge[3];
-  prim_grads<T>(irK, b.E0 + wK[4], vK, q, gv, ge);
-  const T div = gv[0][0] + gv[1][1] + gv[2][2];
-  T tau[3];
+  R[4] = T(0.5) * (hN * qN + hK * qK) - T(0.5) * (l0 * dd[4] + A * ht + B * qt);
+  T fK[4], fN[4];
+  visc_face<T, D>(wK, irK, vK, q, b, fK);
+  visc_face<T, D>(wN, irN, vN, q, b, fN);
+  outK[0] = R[0];
+  outN[0] = -R[0];
 #pragma unroll
-  for (int i = 0; i < 3; ++i) tau[i] = b.mu * (gv[i][D] + gv[D][i]) - (i == D ? b.mu * T(2.0 / 3.0) * div : T(0));
-#pragma unroll
-  for (int i = 0; i < 3; ++i) out[1 + i] -= sgn * tau[i];
-  out[4] -= sgn * (vK[0] * tau[0] + vK[1] * tau[1] + vK[2] * tau[2] + b.kappa * ge[D]);
+  for (int i = 0; i < 4; ++i) {
+    outK[1 + i] = R[1 + i] - fK[i];
+    outN[1 + i] = fN[i] - R[1 + i];
+  }
 }
 
 // ---------------------------------------------------------------------
@@ -340,36 +364,58 @@ __device__ inline double block_sum(double v, double *sw) {
 
 // ---------------------------------------------------------------------
 // Kernel A: auxiliary gradient (solver.cpp:490-508, restated in strong
-// form: -M1^{-1} S1 + face lifts = Dn + M1^{-1} e_0 e_0^T, exactly).
-// Computed in the gradient's own type TQ: FP64, or FP32 in the mixed path
-// (operator entries and the FP32 input fluctuations in FP32 LDS; 7 % faster
-// than FP64 arithmetic at 64^3, p = 4).  One thread per node computes all 15
-// components and stores them straight to HBM (consecutive threads ->
-// consecutive nodes, coalesced).
+// form: -M1^{-1} S1 + face lifts = Dn + M1^{-1} e_0 e_0^T, exactly), and
+// the face fluxes of the elements' negative faces.
+//
+// Gradient: computed in the gradient's own type TQ: FP64, or FP32 in the
+// mixed path (operator entries and the FP32 input fluctuations in FP32 LDS).
+// One thread per node computes all 15 components and stores them straight
+// to HBM (consecutive threads -> consecutive nodes, coalesced).
+//
+// Face fluxes: the face between an element N and its -d neighbour K needs
+// K's trace (already staged here for the gradient's jump term), N's trace
+// and N's gradient trace (the one-sided switch picks it for both sides), so
+// the workgroup that owns N evaluates that face once for both elements:
+// interpolation of the 25 trace fields to the face Gauss points (two
+// sweeps), then Roe + viscous fluxes (face_flux_pair), weighted for the lift
+// (w_a w_b 2/dx_d), written to FF[element][d][side][state][face point] with
+// side 0 = N's own flux on its -d face and side 1 = K's flux on its +d face.
+// The right-hand-side kernel only lifts them.  In type TQ, on fluctuations
+// about the background, like the rest of the flux arithmetic.
 // ---------------------------------------------------------------------
-// Occupancy target of the gradient kernel (waves per SIMD): 6 with an FP32
-// gradient up to p = 4 (measured with FP64 arithmetic: 0.79 vs 0.87 ms at
-// 64^3, p = 4; the FP32 arithmetic needs 58 registers, so 8 fit anyway), 5
-// (the compiler's own choice) with FP64 and at p = 5 (where 6 spilled);
-// never more than its LDS footprint allows (one workgroup = one wave per
-// SIMD, 160 KiB of LDS per CU).
+// Occupancy target of the gradient kernel (waves per SIMD), which sets its
+// register budget: 5 with an FP32 gradient up to p = 4 (the face flux needs
+// more registers than the 80 of 6 waves), 4 with FP64 and at p = 5; never
+// more than its LDS footprint allows (one workgroup = one wave per SIMD,
+// 160 KiB of LDS per CU).
 template <int N, int E, typename TQ>
 struct GradWaves {
-  static constexpr int lds = (int)sizeof(TQ) * (E * NSTATE * N * N * N + E * 3 * NSTATE * N * N + N * (N + 1)) + 8 * 6 * E;
+  static constexpr int NFF = 2 * NSTATE + NGRAD;
+  static constexpr int lds = (int)sizeof(TQ) * (E * NSTATE * N * N * N + E * 3 * NFF * N * N + N * (N + 1) + N) +
+                             (int)sizeof(SymMat<N, TQ>) + 8 * 6 * E;
   static constexpr int by_lds = (160 * 1024) / lds;
-  static constexpr int want = sizeof(TQ) == 4 && N <= 5 ? 6 : 5;
+  static constexpr int want = sizeof(TQ) == 4 && N <= 5 ? 5 : 4;
   static constexpr int value = want < by_lds ? want : by_lds;
 };
 
+// Face-flux array: per element, 3 directions x 2 sides x 5 states x N^2
+// face points.
+template <int N>
+__host__ __device__ constexpr int ff_per_element() { return 3 * 2 * NSTATE * N * N; }
+
 template <int N, int E, typename TQ, typename UT>
-__global__ void __launch_bounds__(NT, (GradWaves<N, E, TQ>::value)) ldg_grad_kernel(const UT *__restrict__ U, TQ *__restrict__ Q,
-                                                        LaunchArgs la, const Ops<N, double> *__restrict__ opp) {
+__global__ void __launch_bounds__(NT, (GradWaves<N, E, TQ>::value))
+ldg_grad_kernel(const UT *__restrict__ U, TQ *__restrict__ Q, TQ *__restrict__ FF, LaunchArgs la,
+                const Ops<N, double> *__restrict__ opp) {
   constexpr int NN = N * N * N, NF = N * N;
+  constexpr int NFF = 2 * NSTATE + NGRAD;   // face fields: own U (0-4), the -d neighbour's U (5-9), own Q (10-24)
   // arithmetic type: FP32 with an FP32 gradient (mixed), FP64 otherwise
   using C = TQ;
   __shared__ C su[E][NSTATE][NN];
-  __shared__ C sj[E][3][NSTATE][NF];
+  __shared__ C sf[E][3][NFF][NF];   // -d face of each element: nodal traces, then Gauss-point values
   __shared__ C sD[N][N + 1];
+  __shared__ C sw[N];
+  __shared__ SymMat<N, C> sI;
   __shared__ long snb[E][6];
   const int lb = logical_block(la);
   if (lb < 0) return;
@@ -389,6 +435,8 @@ __global__ void __launch_bounds__(NT, (GradWaves<N, E, TQ>::value)) ldg_grad_ker
     const int i = tid / (N + 1), j = tid % (N + 1);
     sD[i][j] = C(j < N ? opp->Dn[i][j] + (j == 0 ? opp->L0[i] : 0.0) : -opp->L0[i]);
   }
+  if (tid == 64) convert_symmat(sI, opp->sI);
+  if (tid >= 128 && tid < 128 + N) sw[tid - 128] = C(opp->w[tid - 128]);
   __syncthreads();   // neighbour table
 
   // the block's state (contiguous) and the -d neighbours' traces on their
@@ -401,7 +449,7 @@ __global__ void __launch_bounds__(NT, (GradWaves<N, E, TQ>::value)) ldg_grad_ker
     constexpr int JL = (E * 3 * NSTATE * NF + NT - 1) / NT;
     const UT *Ub = U + k0 * NSTATE * NN;
     const int tot = ne * NSTATE * NN;
-    // trace item w = ((e * 3 + d) * NSTATE + s) * NF + ab is also its index in sj
+    // trace item w = ((e * 3 + d) * NSTATE + s) * NF + ab
     const int totj = ne * 3 * NSTATE * NF;
     UT v[UL], vj[JL];
 #pragma unroll
@@ -438,7 +486,11 @@ __global__ void __launch_bounds__(NT, (GradWaves<N, E, TQ>::value)) ldg_grad_ker
 #pragma unroll
     for (int it = 0; it < JL; ++it) {
       const int w = tid + it * NT;
-      if (w < totj) (&sj[0][0][0][0])[w] = C(in(vj[it], (w / NF) % NSTATE));
+      if (w < totj) {
+        const int e = w / (3 * NSTATE * NF);
+        const int r = w - e * (3 * NSTATE * NF);
+        sf[e][r / (NSTATE * NF)][NSTATE + (r / NF) % NSTATE][r % NF] = C(in(vj[it], (w / NF) % NSTATE));
+      }
     }
   }
   __syncthreads();
@@ -450,21 +502,117 @@ __global__ void __launch_bounds__(NT, (GradWaves<N, E, TQ>::value)) ldg_grad_ker
 #pragma unroll
     for (int j = 0; j <= N; ++j) { rx[j] = sD[ix][j]; ry[j] = sD[iy][j]; rz[j] = sD[iz][j]; }
     TQ *Qe = Q + (k0 + e) * NGRAD * NN + node;
+    C g[NGRAD];
 #pragma unroll
     for (int s = 0; s < NSTATE; ++s) {
       const C *us = su[e][s];
-      C qx = rx[N] * sj[e][0][s][iy + N * iz];
-      C qy = ry[N] * sj[e][1][s][ix + N * iz];
-      C qz = rz[N] * sj[e][2][s][ix + N * iy];
+      C qx = rx[N] * sf[e][0][NSTATE + s][iy + N * iz];
+      C qy = ry[N] * sf[e][1][NSTATE + s][ix + N * iz];
+      C qz = rz[N] * sf[e][2][NSTATE + s][ix + N * iy];
 #pragma unroll
       for (int j = 0; j < N; ++j) {
         qx += rx[j] * us[j + N * iy + NF * iz];
         qy += ry[j] * us[ix + N * j + NF * iz];
         qz += rz[j] * us[ix + N * iy + NF * j];
       }
-      Qe[(0 * NSTATE + s) * NN] = qx * C(m.ialpha[0]);
-      Qe[(1 * NSTATE + s) * NN] = qy * C(m.ialpha[1]);
-      Qe[(2 * NSTATE + s) * NN] = qz * C(m.ialpha[2]);
+      g[0 * NSTATE + s] = qx * C(m.ialpha[0]);
+      g[1 * NSTATE + s] = qy * C(m.ialpha[1]);
+      g[2 * NSTATE + s] = qz * C(m.ialpha[2]);
+      Qe[(0 * NSTATE + s) * NN] = g[0 * NSTATE + s];
+      Qe[(1 * NSTATE + s) * NN] = g[1 * NSTATE + s];
+      Qe[(2 * NSTATE + s) * NN] = g[2 * NSTATE + s];
+    }
+    // the gradient on the element's -d faces, for their fluxes (a node on an
+    // edge or corner belongs to several)
+    if (ix == 0) {
+#pragma unroll
+      for (int k = 0; k < NGRAD; ++k) sf[e][0][2 * NSTATE + k][iy + N * iz] = g[k];
+    }
+    if (iy == 0) {
+#pragma unroll
+      for (int k = 0; k < NGRAD; ++k) sf[e][1][2 * NSTATE + k][ix + N * iz] = g[k];
+    }
+    if (iz == 0) {
+#pragma unroll
+      for (int k = 0; k < NGRAD; ++k) sf[e][2][2 * NSTATE + k][ix + N * iy] = g[k];
+    }
+  }
+  __syncthreads();
+
+  // interpolate the 25 face fields to the face Gauss points: along a (the
+  // own trace is read from the volume state; both traces minus the
+  // background in the FP64 layout, whose state is not stored as a
+  // fluctuation -- exact, the values are close), then along b in place;
+  // face coordinates (a, b) = (y, z), (x, z), (x, y) for d = 0, 1, 2
+  {
+    constexpr bool FLUCT = sizeof(TQ) == 4;   // su holds fluctuations (mixed) or the state (FP64)
+    const C brho = FLUCT ? C(0) : C(d.ph.rho0), bE = FLUCT ? C(0) : C(d.ph.E0);
+    SymReg<N, C> I;
+    I.load(sI);
+    for (int w = tid; w < ne * 3 * NFF * N; w += NT) {
+      const int e = w / (3 * NFF * N);
+      const int r = w - e * (3 * NFF * N);
+      const int dd = r / (NFF * N), f = (r / N) % NFF, b = r % N;
+      C v[N], o[N];
+      if (f < NSTATE) {
+        const C bg = f == 0 ? brho : f == 4 ? bE : C(0);
+#pragma unroll
+        for (int a = 0; a < N; ++a) v[a] = su[e][f][line_base<N>(dd, a, b)] - bg;
+      } else {
+        // the neighbour's trace is a state too (minus the background in the
+        // FP64 layout), the gradient is not
+        const C bg = f == NSTATE ? brho : f == 2 * NSTATE - 1 ? bE : C(0);
+#pragma unroll
+        for (int a = 0; a < N; ++a) v[a] = sf[e][dd][f][a + N * b] - bg;
+      }
+      sym_apply<N, 1, C>(I.m, v, o);
+#pragma unroll
+      for (int q = 0; q < N; ++q) sf[e][dd][f][q + N * b] = o[q];
+    }
+  }
+  __syncthreads();
+  {
+    SymReg<N, C> I;
+    I.load(sI);
+    for (int w = tid; w < ne * 3 * NFF * N; w += NT) {
+      const int e = w / (3 * NFF * N);
+      const int r = w - e * (3 * NFF * N);
+      const int dd = r / (NFF * N), f = (r / N) % NFF, qa = r % N;
+      C v[N], o[N];
+#pragma unroll
+      for (int b = 0; b < N; ++b) v[b] = sf[e][dd][f][qa + N * b];
+      sym_apply<N, 1, C>(I.m, v, o);
+#pragma unroll
+      for (int q = 0; q < N; ++q) sf[e][dd][f][qa + N * q] = o[q];
+    }
+  }
+  __syncthreads();
+
+  // fluxes of both sides at every face Gauss point, weighted for the lift
+  {
+    const Bg<C> bg(d.ph);
+    for (int w = tid; w < ne * 3 * NF; w += NT) {
+      const int e = w / (3 * NF);
+      const int r = w - e * (3 * NF);
+      const int dd = r / NF, ab = r % NF;
+      const C *fp = &sf[e][dd][0][ab];
+      C wN[5], wK[5], q[NGRAD], outK[5], outN[5];
+#pragma unroll
+      for (int s = 0; s < NSTATE; ++s) { wN[s] = fp[s * NF]; wK[s] = fp[(NSTATE + s) * NF]; }
+#pragma unroll
+      for (int k = 0; k < NGRAD; ++k) q[k] = fp[(2 * NSTATE + k) * NF];
+      if (dd == 0) face_flux_pair<C, 0>(wK, wN, q, bg, outK, outN);
+      else if (dd == 1) face_flux_pair<C, 1>(wK, wN, q, bg, outK, outN);
+      else face_flux_pair<C, 2>(wK, wN, q, bg, outK, outN);
+      // (a select, not an index: a run-time index into m would put it in scratch)
+      const double ia = dd == 0 ? m.ialpha[0] : dd == 1 ? m.ialpha[1] : m.ialpha[2];
+      const C wt = sw[ab % N] * sw[ab / N] * C(ia);
+      TQ *dst = FF + (k0 + e) * ff_per_element<N>() + (dd * 2) * NSTATE * NF + ab;
+#pragma unroll
+      for (int s = 0; s < NSTATE; ++s) {
+        dst[s * NF] = outN[s] * wt;                  // side 0: this element, its -d face
+        dst[(NSTATE + s) * NF] = outK[s] * wt;       // side 1: the -d neighbour, its +d face
+      }
     }
   }
 }
@@ -481,16 +629,10 @@ template <int N, typename T>
 struct RhsLds {
   static constexpr int NN = N * N * N, NF = N * N;
   static constexpr int NV = NSTATE + NGRAD;          // 20 volume fields
-  static constexpr int NFF = 25;                     // face fields: uK(5), uN(5), q~(15)
   static constexpr int SV = NV * NN;
-  static constexpr int FB = 2 * NFF * NF;
-  static constexpr int BUF = SV > FB ? SV : FB;      // per element, even for every N
-  // FP64 at p >= 3: each direction's face gather is issued during the
-  // previous direction (registers are free there: occupancy is bound by
-  // LDS), rhs -1 % at p = 3 and 4, -2 % at p = 5; at p = 2 it was 1 % slower
-  // and at p = 1 it spills.  FP32: the extra registers cost a wave per SIMD
-  // at p <= 3 (+7-13 %) and gain nothing at p >= 4.
-  static constexpr bool PREFETCH_FACES = sizeof(T) == 8 && N >= 4;
+  static constexpr int FE = 3 * 2 * NSTATE * NF;     // face fluxes of one element (3 directions, 2 sides)
+  static constexpr int BUF = SV;                     // per element; holds FE first
+  static_assert(FE <= SV, "face fluxes fit the volume buffer");
   // occupancy target (waves per SIMD) handed to the register allocator
   static constexpr int WAVES = sizeof(T) == 4 ? (N <= 5 ? 4 : 2) : (N <= 2 ? 4 : N <= 5 ? 3 : 2);
 };
@@ -530,244 +672,78 @@ struct Mat {
   }
 };
 
-// Per-thread work items of the face phases.  They are the same for the three
-// directions, so with up to four gather lines per thread (p = 3..5 at their
-// default tiling) they are decomposed once per workgroup (constant divisions
-// cost several integer instructions each) and kept packed in registers
-// (p = 3: rhs -5 %); with more lines per thread (p <= 2, many elements per
-// workgroup) and at p = 1 they are decomposed where used, since keeping
-// them costs more registers than it saves.  Each item carries the offset of its
-// line (or point) in the workgroup's face buffer (e BUF + ...), so the LDS
-// phases need no index arithmetic.  Packing (-1: an item past the block):
-//   g  gather / interpolation lines: lds | b << 13 | side << 16 | isQ << 17 | k << 18 | e << 22,
-//      lds = e BUF + slot NF + N b with slot = side NFF + (isQ ? 10 + k : k),
-//      k = trace field (0..4 own U, 5..9 neighbour U) or gradient field
-//      (0..14); the U lines of all elements come first, so that most waves
-//      take one branch of the gather
-//   l  lift lines: lds | c << 13 | side << 16 | s << 17 | e << 20,
-//      lds = e BUF + (side NFF + s) NF + N c
-//   f  face points: lds | a << 13 | b << 16 | side << 19 | e << 20,  lds = e BUF + side NFF NF + a + N b
-template <int N, int E, int BUF>
-struct FaceItems {
-  static constexpr int NU = 2 * NSTATE, NFF = NU + NGRAD;   // trace fields, face fields
-  static constexpr int PU = 2 * NU * N, PQ = 2 * NGRAD * N;  // lines per element
-  static constexpr int GL = (E * (PU + PQ) + NT - 1) / NT;
-  static constexpr int LL = (E * 2 * NSTATE * N + NT - 1) / NT;
-  static constexpr int FL = (E * 2 * N * N + NT - 1) / NT;
-  static constexpr bool KEEP = GL <= 4 && N >= 3;
-  static_assert(E * BUF <= 8192 && E <= 16 && N <= 7, "face item packing");
-  int ne, tid;
-  int gk[KEEP ? GL : 1], lk[KEEP ? LL : 1], fk[KEEP ? FL : 1];
-
-  __device__ static inline int gpack(int e, int side, int isq, int k, int b) {
-    const int slot = side * NFF + (isq ? NU + k : k);
-    return (e * BUF + slot * N * N + N * b) | b << 13 | side << 16 | isq << 17 | k << 18 | e << 22;
-  }
-  __device__ static inline int gcode(int w, int ne) {
-    if (w < E * PU) {
-      const int e = w / PU, r = w - e * PU;
-      return e < ne ? gpack(e, r / (NU * N), 0, (r / N) % NU, r % N) : -1;
-    }
-    if (w < E * (PU + PQ)) {
-      const int w2 = w - E * PU, e = w2 / PQ, r = w2 - e * PQ;
-      return e < ne ? gpack(e, r / (NGRAD * N), 1, (r / N) % NGRAD, r % N) : -1;
-    }
-    return -1;
-  }
-  __device__ static inline int lcode(int w, int ne) {
-    const int e = w / (2 * NSTATE * N), r = w - e * (2 * NSTATE * N);
-    const int side = r / (NSTATE * N), s = (r / N) % NSTATE, c = r % N;
-    return e < ne ? (e * BUF + (side * NFF + s) * N * N + N * c) | c << 13 | side << 16 | s << 17 | e << 20 : -1;
-  }
-  __device__ static inline int fcode(int w, int ne) {
-    const int e = w / (2 * N * N), r = w - e * (2 * N * N);
-    const int side = r / (N * N), ab = r % (N * N);
-    return e < ne ? (e * BUF + side * NFF * N * N + ab) | (ab % N) << 13 | (ab / N) << 16 | side << 19 | e << 20 : -1;
-  }
-  __device__ inline FaceItems(int ne_, int tid_) : ne(ne_), tid(tid_) {
-    if (KEEP) {
-#pragma unroll
-      for (int it = 0; it < GL; ++it) gk[it] = gcode(tid + it * NT, ne);
-#pragma unroll
-      for (int it = 0; it < LL; ++it) lk[it] = lcode(tid + it * NT, ne);
-#pragma unroll
-      for (int it = 0; it < FL; ++it) fk[it] = fcode(tid + it * NT, ne);
-    }
-  }
-  __device__ inline int g(int it) const { return KEEP ? gk[it] : gcode(tid + it * NT, ne); }
-  __device__ inline int l(int it) const { return KEEP ? lk[it] : lcode(tid + it * NT, ne); }
-  __device__ inline int f(int it) const { return KEEP ? fk[it] : fcode(tid + it * NT, ne); }
-};
-
-// Gather the nodal face data of direction D for the block straight from HBM
-// into registers: own and neighbour traces and the gradient trace chosen by
-// the one-sided switch, one line along the face coordinate a per item.
-template <int N, int E, typename T, int D, typename UT>
-__device__ inline void face_gather(const UT *const (*pU)[7], const T *const (*pQ)[7],
-                                   const FaceItems<N, E, RhsLds<N, T>::BUF> &fi,
-                                   T (&v)[FaceItems<N, E, RhsLds<N, T>::BUF>::GL][N], double brho, double bE) {
-  using FI = FaceItems<N, E, RhsLds<N, T>::BUF>;
-  constexpr int NN = N * N * N;
-  constexpr int st_d = D == 0 ? 1 : D == 1 ? N : N * N;
-  constexpr int ast = D == 0 ? N : 1;          // memory stride of the face coordinate a
-  constexpr int bst = D == 2 ? N : N * N;      // memory stride of the face coordinate b
-#pragma unroll
-  for (int it = 0; it < FI::GL; ++it) {
-    const int gi = fi.g(it);
-    if (gi >= 0) {
-      const int e = gi >> 22, side = (gi >> 16) & 1, k = (gi >> 18) & 15, b = (gi >> 13) & 7;
-      if (!(gi & (1 << 17))) {
-        // own trace on this face, or the neighbour's on the opposite face
-        const bool nbr = k >= NSTATE;
-        const int s = nbr ? k - NSTATE : k;
-        const int at = (side ^ nbr) ? (N - 1) * st_d : 0;
-        const UT *src = pU[e][nbr ? 1 + 2 * D + side : 0] + (s * NN + at + b * bst);
-        const double bgs = s == 0 ? brho : s == 4 ? bE : 0.0;
-#pragma unroll
-        for (int a = 0; a < N; ++a) v[it][a] = T((double)src[a * ast] - bgs);
-      } else {
-        // own i_d = 0 slice (side 0) or the +d neighbour's (side 1)
-        const T *src = pQ[e][side ? 2 + 2 * D : 0] + (k * NN + b * bst);
-#pragma unroll
-        for (int a = 0; a < N; ++a) v[it][a] = src[a * ast];
-      }
-    } else {
-      // defined for every item, so that no earlier contents stay live
-#pragma unroll
-      for (int a = 0; a < N; ++a) v[it][a] = T(0);
-    }
-  }
-}
-
-// Face terms of one direction D (both faces of every element in the block):
-// gather the face data into v (or, with PREFETCH_FACES, take what the
-// previous direction gathered there), interpolate to the face Gauss points,
-// evaluate Roe + viscous fluxes, lift them back to the face nodes and
-// subtract them from the residual sr.  With PREFETCH_FACES, the gather of
-// the next direction is issued into v once it is consumed, so its latency is
-// covered by this direction's remaining phases.  pU / pQ: per element, its
-// own (0) and its face neighbours' (1 + face) state and gradient.
-template <int N, int E, typename T, int D, typename UT>
-__device__ inline void rhs_face_dir(const UT *const (*pU)[7], const T *const (*pQ)[7],
-                                    const FaceItems<N, E, RhsLds<N, T>::BUF> &fi,
-                                    T (&v)[FaceItems<N, E, RhsLds<N, T>::BUF>::GL][N],
-                                    const MeshArgs &m, const SOps<N, T> *__restrict__ so, const Bg<T> &bg,
-                                    T (*__restrict__ sbuf)[RhsLds<N, T>::BUF], T (*__restrict__ sr)[NSTATE][N * N * N],
-                                    double brho, double bE) {
-  using FI = FaceItems<N, E, RhsLds<N, T>::BUF>;
-  // where two face buffers fit in an element's volume buffer (p >= 4), the
-  // directions alternate between them, so a direction's first phase need not
-  // wait for the previous direction's last reads
-  constexpr bool DB = 2 * RhsLds<N, T>::FB <= RhsLds<N, T>::BUF;
-  T *const sb = &sbuf[0][0] + (DB && (D & 1) ? RhsLds<N, T>::FB : 0);
-  constexpr int NF = N * N;
-  constexpr int st_d = D == 0 ? 1 : D == 1 ? N : N * N;
-  constexpr int ast = D == 0 ? N : 1;          // memory stride of the face coordinate a
-  constexpr int bst = D == 2 ? N : N * N;      // memory stride of the face coordinate b
-  constexpr bool PF = RhsLds<N, T>::PREFETCH_FACES;
-  T vloc[FI::GL][N];
-  auto &vd = *(PF ? &v : &vloc);       // this direction's gathered lines
-  if constexpr (!PF) face_gather<N, E, T, D, UT>(pU, pQ, fi, vd, brho, bE);
-  // interpolate the gathered lines along a: fb[side][f][qa + N b];
-  // f = uK(0..4), uN(5..9), q~(10..24)
+// Face terms of the block: the weighted face fluxes the gradient kernel
+// wrote (FF, per element [direction][side][state][face point]) -- for each
+// element its own record's side 0 on its -d faces and the +d neighbour's
+// side 1 on its +d faces -- are staged in the volume buffer, lifted with
+// I^T along a (all directions at once) and along b, and subtracted from the
+// face nodes of the residual sr, one direction at a time (a node on an edge
+// or corner belongs to faces of several directions).
+template <int N, int E, typename T>
+__device__ inline void rhs_faces(const T *__restrict__ FF, const long (*snb)[6], long k0, int ne,
+                                 const SOps<N, T> *__restrict__ so, T (*__restrict__ sbuf)[RhsLds<N, T>::BUF],
+                                 T (*__restrict__ sr)[NSTATE][N * N * N], int tid) {
+  constexpr int NF = N * N, NS = NSTATE * NF;
+  constexpr int FE = RhsLds<N, T>::FE;
+  constexpr int FL = (E * FE + NT - 1) / NT;
+  // loads of all directions first (a record is 2 NS contiguous values per
+  // direction: the own one for side 0, the +d neighbour's for side 1)
   {
-    SymReg<N, T> I;
-    I.load(so->I);
+    T v[FL];
 #pragma unroll
-    for (int it = 0; it < FI::GL; ++it) {
-      const int gi = fi.g(it);
-      if (gi >= 0) {
-        T *dst = sb + (gi & 8191);
-        T o[N];
-        sym_apply<N, 1, T>(I.m, vd[it], o);
-#pragma unroll
-        for (int q = 0; q < N; ++q) dst[q] = o[q];
-      }
+    for (int it = 0; it < FL; ++it) {
+      const int w = tid + it * NT;
+      const int e = w / FE, r = w - e * FE;
+      const int dd = r / (2 * NS), side = (r / NS) & 1;
+      const long K = side ? snb[e < ne ? e : 0][2 * dd + 1] : k0 + e;
+      v[it] = e < ne ? FF[K * FE + r] : T(0);
     }
-  }
-  if constexpr (PF && D < 2) face_gather<N, E, T, D + 1, UT>(pU, pQ, fi, v, brho, bE);
-  __syncthreads();
-  // interpolate along b (in place)
-  {
-    SymReg<N, T> I;
-    I.load(so->I);
 #pragma unroll
-    for (int it = 0; it < FI::GL; ++it) {
-      const int gi = fi.g(it);
-      if (gi >= 0) {
-        const int qa = (gi >> 13) & 7;
-        T *p = sb + ((gi & 8191) - (N - 1) * qa);
-        T v[N], o[N];
-#pragma unroll
-        for (int b = 0; b < N; ++b) v[b] = p[b * N];
-        sym_apply<N, 1, T>(I.m, v, o);
-#pragma unroll
-        for (int q = 0; q < N; ++q) p[q * N] = o[q];
-      }
+    for (int it = 0; it < FL; ++it) {
+      const int w = tid + it * NT;
+      const int e = w / FE, r = w - e * FE;
+      if (e < ne) sbuf[e][r] = v[it];
     }
   }
   __syncthreads();
-  // flux at each face Gauss point, weighted for the lift
-#pragma unroll
-  for (int it = 0; it < FI::FL; ++it) {
-    const int fp = fi.f(it);
-    if (fp >= 0) {
-      const int side = (fp >> 19) & 1;
-      T *fb = sb + (fp & 8191);
-      T uK[5], uN[5], qq[15], out[5];
-#pragma unroll
-      for (int s = 0; s < 5; ++s) { uK[s] = fb[s * NF]; uN[s] = fb[(5 + s) * NF]; }
-#pragma unroll
-      for (int k = 0; k < 15; ++k) qq[k] = fb[(10 + k) * NF];
-      face_flux<T, D>(uK, uN, qq, side ? T(1) : T(-1), bg, out);
-      const T wt = so->w[(fp >> 13) & 7] * so->w[(fp >> 16) & 7] * T(m.ialpha[D]);
-#pragma unroll
-      for (int s = 0; s < 5; ++s) fb[s * NF] = out[s] * wt;
-    }
-  }
-  __syncthreads();
-  // lift: I^T along a (in place)
+  // I^T along a, in place: lines of N values at (direction, side, state, qb)
   {
     SymReg<N, T> It;
     It.load(so->It);
+    for (int w = tid; w < ne * (FE / N); w += NT) {
+      const int e = w / (FE / N), r = w - e * (FE / N);
+      T *p = &sbuf[e][N * r];
+      T v[N], o[N];
 #pragma unroll
-    for (int it = 0; it < FI::LL; ++it) {
-      const int li = fi.l(it);
-      if (li >= 0) {
-        T *p = sb + (li & 8191);
-        T v[N], o[N];
+      for (int q = 0; q < N; ++q) v[q] = p[q];
+      sym_apply<N, 1, T>(It.m, v, o);
 #pragma unroll
-        for (int q = 0; q < N; ++q) v[q] = p[q];
-        sym_apply<N, 1, T>(It.m, v, o);
-#pragma unroll
-        for (int i = 0; i < N; ++i) p[i] = o[i];
-      }
+      for (int i = 0; i < N; ++i) p[i] = o[i];
     }
   }
   __syncthreads();
   // I^T along b, subtracted into the face nodes of the residual
-  {
-    SymReg<N, T> It;
-    It.load(so->It);
+  SymReg<N, T> It;
+  It.load(so->It);
 #pragma unroll
-    for (int it = 0; it < FI::LL; ++it) {
-      const int li = fi.l(it);
-      if (li >= 0) {
-        const int a = (li >> 13) & 7, side = (li >> 16) & 1, s = (li >> 17) & 7, e = li >> 20;
-        const T *p = sb + ((li & 8191) - (N - 1) * a);
-        T v[N], o[N];
+  for (int dd = 0; dd < 3; ++dd) {
+    const int st_d = dd == 0 ? 1 : dd == 1 ? N : N * N;
+    const int ast = dd == 0 ? N : 1;          // memory stride of the face coordinate a
+    const int bst = dd == 2 ? N : N * N;      // memory stride of the face coordinate b
+    for (int w = tid; w < ne * 2 * NSTATE * N; w += NT) {
+      const int e = w / (2 * NSTATE * N), r = w - e * (2 * NSTATE * N);
+      const int side = r / (NSTATE * N), s = (r / N) % NSTATE, a = r % N;
+      const T *p = &sbuf[e][(dd * 2 + side) * NS + s * NF + a];
+      T v[N], o[N];
 #pragma unroll
-        for (int q = 0; q < N; ++q) v[q] = p[q * N];
-        sym_apply<N, 1, T>(It.m, v, o);
-        T *dst = &sr[e][s][a * ast + (side ? (N - 1) * st_d : 0)];
+      for (int q = 0; q < N; ++q) v[q] = p[q * N];
+      sym_apply<N, 1, T>(It.m, v, o);
+      T *dst = &sr[e][s][a * ast + (side ? (N - 1) * st_d : 0)];
 #pragma unroll
-        for (int b = 0; b < N; ++b) dst[b * bst] -= o[b];
-      }
+      for (int b = 0; b < N; ++b) dst[b * bst] -= o[b];
     }
+    __syncthreads();
   }
-  // the next direction writes the other face buffer; the lifts of two
-  // directions (both update sr) stay separated by its four barriers
-  if (!DB || D == 2) __syncthreads();
 }
 
 // Work mapping of the volume-line loops (E elements x X = 20 fields x N^2
@@ -797,7 +773,7 @@ struct VolMap {
 
 template <int N, int E, typename T, typename UT>
 __global__ void __launch_bounds__(NT, (RhsLds<N, T>::WAVES))
-ldg_rhs_kernel(const UT *__restrict__ U, const T *__restrict__ Q, LaunchArgs la,
+ldg_rhs_kernel(const UT *__restrict__ U, const T *__restrict__ Q, const T *__restrict__ FF, LaunchArgs la,
                const Ops<N, double> *__restrict__ opp, StageArgs st) {
   using L = RhsLds<N, T>;
   constexpr int NN = L::NN, NF = L::NF, BUF = L::BUF;
@@ -806,8 +782,6 @@ ldg_rhs_kernel(const UT *__restrict__ U, const T *__restrict__ Q, LaunchArgs la,
   __shared__ double sred[NT / 64];
   __shared__ SOps<N, T> sops;
   __shared__ long snb[E][6];
-  __shared__ const UT *pU[E][7];      // per element: own (0) and face neighbours' (1 + face) state
-  __shared__ const T *pQ[E][7];       // and gradient
   const int lb = logical_block(la);
   if (lb < 0) return;
   MeshArgs m;
@@ -839,24 +813,10 @@ ldg_rhs_kernel(const UT *__restrict__ U, const T *__restrict__ Q, LaunchArgs la,
   fill_neighbors(m, ec, d.elem_off, ne, snb, tid);
   load_sops<N, T>(&sops, opp, tid);
   for (int w = tid; w < ne * NSTATE * NN; w += NT) (&sr[0][0][0])[w] = T(0);
-  const FaceItems<N, E, BUF> fi(ne, tid);
-  __syncthreads();
-  if (tid < ne * 7) {
-    const int e = tid / 7, f = tid % 7;
-    const long K = f ? snb[e][f - 1] : k0 + e;
-    pU[e][f] = U + K * (NSTATE * NN);
-    pQ[e][f] = Q + K * (NGRAD * NN);
-  }
   __syncthreads();
 
-  // ================= faces, one direction at a time =================
-  {
-    T fv[FaceItems<N, E, BUF>::GL][N];
-    if constexpr (L::PREFETCH_FACES) face_gather<N, E, T, 0, UT>(pU, pQ, fi, fv, brho, bE);
-    rhs_face_dir<N, E, T, 0, UT>(pU, pQ, fi, fv, m, so, bg, sbuf, sr, brho, bE);
-    rhs_face_dir<N, E, T, 1, UT>(pU, pQ, fi, fv, m, so, bg, sbuf, sr, brho, bE);
-    rhs_face_dir<N, E, T, 2, UT>(pU, pQ, fi, fv, m, so, bg, sbuf, sr, brho, bE);
-  }
+  // ================= faces: lift the gradient kernel's fluxes =================
+  rhs_faces<N, E, T>(FF, snb, k0, ne, so, sbuf, sr, tid);
 
   // ================= volume =================
   // load U (minus the background) and Q x-lines from HBM, all loads of the
@@ -891,12 +851,12 @@ ldg_rhs_kernel(const UT *__restrict__ U, const T *__restrict__ Q, LaunchArgs la,
       if (e < ne && r < VM::X) {
         const int o = line_start(it, 0, r);   // f NN + N (y, z line index): also the line's offset in HBM
         if (o < NSTATE * NN) {
-          const UT *src = (VM::PER_E ? U + (k0 + e) * (NSTATE * NN) : pU[e][0]) + o;
+          const UT *src = U + (k0 + e) * (NSTATE * NN) + o;
           const double bgs = o < NN ? brho : o >= 4 * NN ? bE : 0.0;
 #pragma unroll
           for (int i = 0; i < N; ++i) v[it][i] = T((double)src[i] - bgs);
         } else {
-          const T *src = (VM::PER_E ? Q + (k0 + e) * (NGRAD * NN) : pQ[e][0]) + (o - NSTATE * NN);
+          const T *src = Q + (k0 + e) * (NGRAD * NN) + (o - NSTATE * NN);
 #pragma unroll
           for (int i = 0; i < N; ++i) v[it][i] = src[i];
         }
