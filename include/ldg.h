@@ -62,8 +62,8 @@ int ldg_create(const ldg_config *cfg, ldg_handle **out);
 /* Releases device memory (Solver's destructor). */
 void ldg_destroy(ldg_handle *h);
 
-/* Last error message of the handle (or of the last failed create when
-   h == NULL). */
+/* Last error message of the handle (or, when h == NULL, of the last failed
+   call without a handle: ldg_create, ldg_ensemble_create, ldg_rhs_noalloc). */
 const char *ldg_last_error(const ldg_handle *h);
 
 /* dofs = (p+1)^3 and the element count K (Solver::dofs, mesh.nElements). */
@@ -89,6 +89,27 @@ double ldg_time_step(const ldg_handle *h, double cfl);
    solver.cpp:486-535).  u, k are host arrays (dofs,5,K); Du, if not NULL,
    receives the auxiliary gradient Dus (dofs,5,K,3). */
 int ldg_rhs(ldg_handle *h, const double *u, double *k, double *Du);
+
+/* Device scratch of ldg_rhs_noalloc per GPU, in bytes. */
+#define LDG_NOALLOC_SCRATCH_BYTES (64u << 20)
+
+/* k = rhs(u) exactly as ldg_rhs computes it, for callers that cannot
+   allocate device memory: code under HIP graph capture, tight serving
+   loops, harnesses that forbid hipMalloc.  No counterpart in the reference
+   (its rk4Rhs, solver.cpp:486-535, allocates scratch on every call).
+   u and k are DEVICE arrays (dofs,5,K) in the layout above; the mesh,
+   physics and precision come from cfg; there is no handle.  The operators,
+   instance tables, auxiliary gradient and face fluxes live in one fixed
+   device array of LDG_NOALLOC_SCRATCH_BYTES per GPU (held until the process
+   ends), so the mesh must satisfy
+   8192 + w K (15 (p+1)^3 + 30 (p+1)^2) <= LDG_NOALLOC_SCRATCH_BYTES with
+   w = 8 (LDG_FP64) or 4 (LDG_MIXED) bytes -- e.g. up to 14^3 elements at
+   p = 4 in FP64 -- and a larger one returns LDG_EINVAL.  Runs on `stream`
+   (a hipStream_t; NULL: the default stream) and is complete when it
+   returns; calls no hipMalloc/hipFree.  All calls on one GPU share its
+   array: calls that overlap in time on one GPU (several threads or
+   streams) race, and must be serialised by the caller. */
+int ldg_rhs_noalloc(const ldg_config *cfg, const double *u, double *k, void *stream);
 
 /* nsteps RK4 steps of size dt on the resident state (the step loop of
    Solver::dgTimeStep, solver.cpp:347-442).  If ke != NULL it receives the

@@ -102,6 +102,7 @@ namespace ldg_host {
 // Records the message on the handle (or as the last create error) and
 // returns code.
 int fail(ldg_handle *h, int code, const std::string &msg);
+int fail(ldg_handle *h, int code, const char *msg);   // records without a heap copy when h is null
 
 #define HIPCHECK(h, call)                                                                         \
   do {                                                                                            \

@@ -23,6 +23,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <random>
 #include <string>
 #include <vector>
@@ -33,10 +34,19 @@ using namespace ldg;
 
 namespace ldg_host {
 
-std::string g_create_error;
+// message of the last failed call without a handle (create, allocation-free
+// right-hand side); a fixed buffer, so that recording it allocates nothing
+char g_create_error[512] = "";
+
+void set_global_error(const char *msg) { std::snprintf(g_create_error, sizeof(g_create_error), "%s", msg); }
 
 int fail(ldg_handle *h, int code, const std::string &msg) {
-  if (h) h->err = msg; else g_create_error = msg;
+  if (h) h->err = msg; else set_global_error(msg.c_str());
+  return code;
+}
+
+int fail(ldg_handle *h, int code, const char *msg) {
+  if (h) h->err = msg; else set_global_error(msg);
   return code;
 }
 
@@ -318,7 +328,7 @@ int create_impl(const ldg_config *cfg, const ldg_instance *insts, int n, ldg_han
   ldg_handle *h = new ldg_handle();
   h->cfg = *cfg;
   h->mixed = cfg->precision == LDG_MIXED;
-  auto bad = [&](int code) { g_create_error = h->err; ldg_destroy(h); return code; };
+  auto bad = [&](int code) { set_global_error(h->err.c_str()); ldg_destroy(h); return code; };
   for (int i = 0; i < n; ++i) {
     Inst in;
     in.p = insts[i].p; in.nx = insts[i].nx; in.ny = insts[i].ny; in.nz = insts[i].nz;
@@ -677,7 +687,7 @@ int ldg_config_default(ldg_config *c) {
   return LDG_OK;
 }
 
-const char *ldg_last_error(const ldg_handle *h) { return h ? h->err.c_str() : g_create_error.c_str(); }
+const char *ldg_last_error(const ldg_handle *h) { return h ? h->err.c_str() : g_create_error; }
 
 int ldg_create(const ldg_config *cfg, ldg_handle **out) {
   if (!cfg || !out) return fail(nullptr, LDG_EINVAL, "null argument");
@@ -1034,3 +1044,171 @@ int ldg_kernel_times(ldg_handle *h, double dt, int nsteps, double ms[2]) {
 }
 
 }  // extern "C"
+
+// ------------------------------------------------- allocation-free right-hand side
+//
+// ldg_rhs_noalloc keeps everything beyond its input and output in one fixed
+// device array per GPU: the 1D operators, the two instance tables (gradient
+// and right-hand-side tilings), the auxiliary gradient Q and the face fluxes.  Host code never
+// uses the array's name as an address: its device address is looked up once
+// per device with hipGetSymbolAddress and kept in g_scratch_ptr.
+static __device__ __align__(256) unsigned char scratch[LDG_NOALLOC_SCRATCH_BYTES];
+
+namespace ldg_host {
+
+constexpr size_t kScrOps = 0, kScrInstGrad = 4096, kScrInstRhs = 6144, kScrQ = 8192;
+static_assert(sizeof(Ops<6, double>) <= kScrInstGrad - kScrOps, "operators do not fit their scratch slot");
+static_assert(sizeof(InstDesc) <= kScrInstRhs - kScrInstGrad, "instance table does not fit its scratch slot");
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char *g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of this device's scratch array (looked up on the first call
+// on the device), or nullptr with the error recorded.
+unsigned char *scratch_on(int dev) {
+  if (dev < 0 || dev >= kMaxDevices) {
+    set_global_error("ldg_rhs_noalloc: device index out of range");
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      char msg[256];
+      std::snprintf(msg, sizeof(msg), "ldg_rhs_noalloc: hipGetSymbolAddress: %s", hipGetErrorString(e));
+      set_global_error(msg);
+      return nullptr;
+    }
+    g_scratch_ptr[dev] = static_cast<unsigned char *>(p);
+  }
+  return g_scratch_ptr[dev];
+}
+
+// The 1D operators of every order, built on the host once per process.
+struct OpsTables {
+  Ops<2, double> o2;
+  Ops<3, double> o3;
+  Ops<4, double> o4;
+  Ops<5, double> o5;
+  Ops<6, double> o6;
+  bool ok = true;
+};
+const OpsTables &ops_tables() {
+  static const OpsTables t = [] {
+    OpsTables r;
+    double a = 0.0;
+    r.o2 = make_ops<2>(make_basis(1), a); r.ok = r.ok && a < 1e-9;
+    r.o3 = make_ops<3>(make_basis(2), a); r.ok = r.ok && a < 1e-9;
+    r.o4 = make_ops<4>(make_basis(3), a); r.ok = r.ok && a < 1e-9;
+    r.o5 = make_ops<5>(make_basis(4), a); r.ok = r.ok && a < 1e-9;
+    r.o6 = make_ops<6>(make_basis(5), a); r.ok = r.ok && a < 1e-9;
+    return r;
+  }();
+  return t;
+}
+template <int N> const Ops<N, double> &ops_of(const OpsTables &t);
+template <> const Ops<2, double> &ops_of<2>(const OpsTables &t) { return t.o2; }
+template <> const Ops<3, double> &ops_of<3>(const OpsTables &t) { return t.o3; }
+template <> const Ops<4, double> &ops_of<4>(const OpsTables &t) { return t.o4; }
+template <> const Ops<5, double> &ops_of<5>(const OpsTables &t) { return t.o5; }
+template <> const Ops<6, double> &ops_of<6>(const OpsTables &t) { return t.o6; }
+
+// The gradient and right-hand-side kernels of ldg_rhs (default tilings, FP64
+// input) with every intermediate in scr.  The host copies of the operators
+// and instance tables are read by the copies before the stream is
+// synchronised (the caller synchronises before returning).
+template <int N>
+hipError_t noalloc_launch(const ldg_config &cfg, const Inst &in, unsigned char *scr, const double *u, double *k,
+                          hipStream_t s, InstDesc dsc[2]) {
+  constexpr int EG = Cand<N>::grad[0], ER = Cand<N>::rhs[0];
+  const Ops<N, double> &o = ops_of<N>(ops_tables());
+  if (inst_desc(cfg, in, EG, 0, 0, dsc[0]) != LDG_OK || inst_desc(cfg, in, ER, 0, 0, dsc[1]) != LDG_OK)
+    return hipErrorInvalidValue;
+  hipError_t e = hipMemcpyAsync(scr + kScrOps, &o, sizeof(o), hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(scr + kScrInstGrad, &dsc[0], sizeof(InstDesc), hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(scr + kScrInstRhs, &dsc[1], sizeof(InstDesc), hipMemcpyHostToDevice, s);
+  if (e != hipSuccess) return e;
+  const Ops<N, double> *op = reinterpret_cast<const Ops<N, double> *>(scr + kScrOps);
+  const LaunchArgs lg{reinterpret_cast<const InstDesc *>(scr + kScrInstGrad), nullptr, dsc[0].nblocks,
+                      (dsc[0].nblocks + 7) / 8};
+  const LaunchArgs lr{reinterpret_cast<const InstDesc *>(scr + kScrInstRhs), nullptr, dsc[1].nblocks,
+                      (dsc[1].nblocks + 7) / 8};
+  StageArgs st = empty_stage();
+  st.kout = k;
+  const size_t nq = (size_t)in.K * NGRAD * N * N * N;   // gradient values; the face fluxes follow
+  if (cfg.precision == LDG_MIXED) {
+    float *q = reinterpret_cast<float *>(scr + kScrQ), *ff = q + nq;
+    hipLaunchKernelGGL((ldg_grad_kernel<N, EG, float, double>), dim3(8 * lg.per_xcd), dim3(NT), 0, s, u, q, ff, lg,
+                       op);
+    hipLaunchKernelGGL((ldg_rhs_kernel<N, ER, float, double>), dim3(8 * lr.per_xcd), dim3(NT), 0, s, u,
+                       (const float *)q, (const float *)ff, lr, op, st);
+  } else {
+    double *q = reinterpret_cast<double *>(scr + kScrQ), *ff = q + nq;
+    hipLaunchKernelGGL((ldg_grad_kernel<N, EG, double, double>), dim3(8 * lg.per_xcd), dim3(NT), 0, s, u, q, ff, lg,
+                       op);
+    hipLaunchKernelGGL((ldg_rhs_kernel<N, ER, double, double>), dim3(8 * lr.per_xcd), dim3(NT), 0, s, u,
+                       (const double *)q, (const double *)ff, lr, op, st);
+  }
+  return hipGetLastError();
+}
+
+int noalloc_fail(int code, const char *msg) {
+  set_global_error(msg);
+  return code;
+}
+
+}  // namespace ldg_host
+
+extern "C" int ldg_rhs_noalloc(const ldg_config *cfg, const double *u, double *k, void *stream) {
+  using namespace ldg_host;
+  if (!cfg || !u || !k) return noalloc_fail(LDG_EINVAL, "ldg_rhs_noalloc: null argument");
+  const ldg_instance li = default_instance(*cfg);
+  if (validate_instance(*cfg, li) != LDG_OK) return LDG_EINVAL;   // message recorded
+  Inst in;
+  in.p = li.p; in.nx = li.nx; in.ny = li.ny; in.nz = li.nz;
+  in.Re = li.Re; in.M0 = li.M0; in.V0 = li.V0;
+  const int ns[3] = {in.nx, in.ny, in.nz};
+  for (int l = 0; l < 3; ++l) {
+    in.phase[l] = 0.0;
+    in.dx[l] = (cfg->xmax[l] - cfg->xmin[l]) / ns[l];
+  }
+  in.K = (long)in.nx * in.ny * in.nz;
+  in.group = 0;
+  in.slot = 0;
+  const int n = in.p + 1;
+  // the gradient (15 values per node) and the face fluxes (30 per face point)
+  const double qbytes = (double)in.K * (NGRAD * n * n * n + 30 * n * n) * (cfg->precision == LDG_MIXED ? 4.0 : 8.0);
+  if (kScrQ + qbytes > (double)LDG_NOALLOC_SCRATCH_BYTES) {
+    char msg[256];
+    std::snprintf(msg, sizeof(msg),
+                  "ldg_rhs_noalloc: %d x %d x %d elements at p = %d need %.0f bytes of device scratch; the limit is "
+                  "%u (LDG_NOALLOC_SCRATCH_BYTES)", in.nx, in.ny, in.nz, in.p, kScrQ + qbytes,
+                  (unsigned)LDG_NOALLOC_SCRATCH_BYTES);
+    return noalloc_fail(LDG_EINVAL, msg);
+  }
+  if (!ops_tables().ok) return noalloc_fail(LDG_EINVAL, "ldg_rhs_noalloc: 1D operators are not centro-symmetric");
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return noalloc_fail(LDG_EDEVICE, "ldg_rhs_noalloc: no HIP device");
+  unsigned char *scr = scratch_on(dev);
+  if (!scr) return LDG_EDEVICE;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  InstDesc dsc[2];
+  hipError_t e = hipErrorInvalidValue;
+  switch (n) {
+    case 2: e = noalloc_launch<2>(*cfg, in, scr, u, k, s, dsc); break;
+    case 3: e = noalloc_launch<3>(*cfg, in, scr, u, k, s, dsc); break;
+    case 4: e = noalloc_launch<4>(*cfg, in, scr, u, k, s, dsc); break;
+    case 5: e = noalloc_launch<5>(*cfg, in, scr, u, k, s, dsc); break;
+    case 6: e = noalloc_launch<6>(*cfg, in, scr, u, k, s, dsc); break;
+  }
+  const hipError_t es = hipStreamSynchronize(s);
+  if (e == hipSuccess) e = es;
+  if (e != hipSuccess) {
+    char msg[256];
+    std::snprintf(msg, sizeof(msg), "ldg_rhs_noalloc: %s", hipGetErrorString(e));
+    return noalloc_fail(LDG_EDEVICE, msg);
+  }
+  return LDG_OK;
+}

@@ -38,7 +38,7 @@ EXPORTED = [
     "ldg_closure_get", "ldg_closure_set", "ldg_closure_forward", "ldg_closure_gradient",
     "ldg_project_state", "ldg_closure_collect", "ldg_closure_add_samples", "ldg_closure_samples",
     "ldg_closure_get_samples", "ldg_closure_fit_normalization", "ldg_closure_train", "ldg_set_closure",
-    "ldg_closure_gated", "ldg_closure_set_weighting", "ldg_closure_get_weights",
+    "ldg_closure_gated", "ldg_closure_set_weighting", "ldg_closure_get_weights", "ldg_rhs_noalloc",
 ]
 
 
@@ -93,6 +93,7 @@ def load_library(path=LIB_PATH):
     L.ldg_time_step.argtypes = [h, ctypes.c_double]
     L.ldg_time_step.restype = ctypes.c_double
     L.ldg_rhs.argtypes = [h, _dp, _dp, _dp]
+    L.ldg_rhs_noalloc.argtypes = [ctypes.POINTER(ldg_config), ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
     L.ldg_step.argtypes = [h, ctypes.c_double, ctypes.c_int, _dp]
     L.ldg_diagnostics.argtypes = [h, _dp]
     L.ldg_kernel_times.argtypes = [h, ctypes.c_double, ctypes.c_int, _dp]
@@ -243,6 +244,17 @@ class LDG:
         self._check(self._lib.ldg_autotune(self._h, out))
         ng = len({p for p in getattr(self, "orders", [self.p])})
         return [tuple(out[3 * g:3 * g + 3]) for g in range(ng)]
+
+
+def rhs_noalloc(cfg, u_dev, k_dev, stream=None):
+    """k = rhs(u) on device arrays (addresses u_dev, k_dev: (K, 5, dofs)
+    FP64), with no device allocation (ldg_rhs_noalloc); cfg is an
+    ldg_config (LDG(...).cfg)."""
+    lib = load_library()
+    rc = lib.ldg_rhs_noalloc(ctypes.byref(cfg), ctypes.c_void_p(u_dev), ctypes.c_void_p(k_dev),
+                             ctypes.c_void_p(stream) if stream else None)
+    if rc != LDG_OK:
+        raise LDGError(rc, lib.ldg_last_error(None).decode())
 
 
 def sample_instances(seed, n):

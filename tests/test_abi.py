@@ -84,3 +84,30 @@ def test_ensemble_create_rejects_bad_instance():
         ldg_amd.Ensemble(bad)
     with pytest.raises(ldg_amd.LDGError):
         ldg_amd.Ensemble([{"p": 2, "nx": 2}])
+
+
+def _hip_devices():
+    try:
+        hip = ctypes.CDLL("libamdhip64.so")
+    except OSError:
+        return 0
+    n = ctypes.c_int(0)
+    return n.value if hip.hipGetDeviceCount(ctypes.byref(n)) == 0 else 0
+
+
+def test_noalloc_rejects_bad_arguments_before_any_device_work():
+    """ldg_rhs_noalloc checks its arguments and the scratch limit on the host,
+    before it looks for a device (so these run without a GPU; with one, the
+    placeholder pointers below are not used)."""
+    if _hip_devices() > 0:
+        pytest.skip("a HIP device is visible: the GPU test checks these paths with device buffers")
+    lib = ldg_amd.load_library()
+    cfg = ldg_amd.ldg_config()
+    lib.ldg_config_default(ctypes.byref(cfg))
+    assert lib.ldg_rhs_noalloc(None, None, None, None) == ldg_amd.LDG_EINVAL
+    cfg.p, cfg.nx, cfg.ny, cfg.nz = 4, 15, 15, 15          # beyond LDG_NOALLOC_SCRATCH_BYTES in FP64
+    dummy = ctypes.c_void_p(8)                              # never dereferenced: refused first
+    assert lib.ldg_rhs_noalloc(ctypes.byref(cfg), dummy, dummy, None) == ldg_amd.LDG_EINVAL
+    assert b"LDG_NOALLOC_SCRATCH_BYTES" in lib.ldg_last_error(None)
+    cfg.nx = 2                                              # thinner than the reference allows
+    assert lib.ldg_rhs_noalloc(ctypes.byref(cfg), dummy, dummy, None) == ldg_amd.LDG_EINVAL
