@@ -180,15 +180,19 @@ def dissipation_err(ke, ke_ref, dt):
     return float(np.max(np.abs(d - d_ref)) / np.max(np.abs(d_ref)))
 
 
-def timed_run(ldg_amd, p, nx, prec, steps, warmup):
+def timed_run(ldg_amd, p, nx, prec, steps, warmup, autotune=True):
+    """Both paths are run alike: tilings autotuned on the initial state
+    (each path gets its fastest elements-per-workgroup choices), warm-up
+    steps, then `steps` timed RK4 steps in one call."""
     g = ldg_amd.LDG(p, nx, precision=prec)
     g.initial_condition()
+    tiling = g.autotune() if autotune else None
     dt = g.time_step(0.01)
     ke_w = g.step(dt, warmup, record_ke=True)
     t0 = time.perf_counter()
     ke = g.step(dt, steps, record_ke=True)          # synchronises on return
     wall = time.perf_counter() - t0
-    return g, dt, wall, np.concatenate([ke_w, ke])
+    return g, dt, wall, np.concatenate([ke_w, ke]), tiling
 
 
 def ensemble_bench(ldg_amd, args):
@@ -280,6 +284,7 @@ def main():
     ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed",
                     help="path reported as `value` (both are timed)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-autotune", action="store_true", help="default tilings instead of autotuned ones")
     ap.add_argument("--cpu-nx", type=int, default=32)
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--cpu1-nx", type=int, default=12, help="mesh of the one-thread CPU sample")
@@ -359,11 +364,11 @@ def main():
 
     runs = {}
     for name, prec in (("fp64", ldg_amd.LDG_FP64), ("mixed", ldg_amd.LDG_MIXED)):
-        g, dt, wall, ke = timed_run(ldg_amd, p, nx, prec, args.steps, args.warmup)
+        g, dt, wall, ke, tiling = timed_run(ldg_amd, p, nx, prec, args.steps, args.warmup, not args.no_autotune)
         dof = g.K * g.dofs * 5
         u = g.get_state()
         kt = g.kernel_times(dt, 4)          # per-kernel device time, HIP events on the library's stream
-        runs[name] = dict(K=g.K, dt=dt, wall=wall, ke=ke, dof=dof, kt=kt, u=u)
+        runs[name] = dict(K=g.K, dt=dt, wall=wall, ke=ke, dof=dof, kt=kt, u=u, tiling=tiling)
         g.close()
 
     main_name = args.precision
@@ -410,7 +415,9 @@ def main():
                                        "alg_GBs": alg_bytes(dof, name == "mixed")[0] / (v["kt"]["grad_ms"] * 1e-3) / 1e9},
                    "ldg_rhs_kernel": {"avg_ms": v["kt"]["rhs_ms"],
                                       "alg_GBs": alg_bytes(dof, name == "mixed")[1] / (v["kt"]["rhs_ms"] * 1e-3) / 1e9},
-                   "ms_per_step": v["wall"] / args.steps * 1e3}
+                   "ms_per_step": v["wall"] / args.steps * 1e3,
+                   "tiling": {"p": v["tiling"][0][0], "grad_elements_per_workgroup": v["tiling"][0][1],
+                              "rhs_elements_per_workgroup": v["tiling"][0][2]} if v["tiling"] else "default"}
             for name, v in runs.items()},
     }
     if not args.no_cpu:

@@ -459,14 +459,15 @@ ldg_grad_kernel(const UT *__restrict__ U, TQ *__restrict__ Q, TQ *__restrict__ F
     }
 #pragma unroll
     for (int it = 0; it < JL; ++it) {
-      const int w = tid + it * NT < totj ? tid + it * NT : totj - 1;
-      const int e = w / (3 * NSTATE * NF);
-      const int r = w - e * (3 * NSTATE * NF);
-      const int d = r / (NSTATE * NF);
-      const int s = (r / NF) % NSTATE;
-      const int ab = r % NF;
-      const int base = line_base<N>(d, ab % N, ab / N), st = line_stride<N>(d);
-      vj[it] = U[snb[e][2 * d] * (NSTATE * NN) + (s * NN + base + (N - 1) * st)];
+      // unsigned: a division by a constant is a multiply-high and a shift
+      const unsigned w = tid + it * NT < totj ? tid + it * NT : totj - 1;
+      const unsigned ed = w / (NSTATE * NF), r = w - ed * (NSTATE * NF);   // ed = e * 3 + d
+      const unsigned e = ed / 3, d = ed - 3 * e;
+      const unsigned s = r / NF, ab = r - s * NF;
+      const unsigned b = ab / N, a = ab - b * N;
+      // node (N-1, a, b), (a, N-1, b) or (a, b, N-1) of the -d neighbour: its +d face
+      const unsigned node = (d == 0 ? N : 1) * a + (d == 2 ? N : NF) * b + (d == 0 ? 1 : d == 1 ? N : NF) * (N - 1);
+      vj[it] = U[snb[e][2 * d] * (NSTATE * NN) + (s * NN + node)];
     }
     // an FP64 input in the mixed path (RK stage 1) is rounded to the FP32
     // fluctuation the other stages read, so a step's result does not depend
@@ -480,24 +481,29 @@ ldg_grad_kernel(const UT *__restrict__ U, TQ *__restrict__ Q, TQ *__restrict__ F
     };
 #pragma unroll
     for (int it = 0; it < UL; ++it) {
-      const int w = tid + it * NT;
-      if (w < tot) (&su[0][0][0])[w] = C(in(v[it], (w / NN) % NSTATE));
+      const unsigned w = tid + it * NT;
+      if (w < (unsigned)tot) (&su[0][0][0])[w] = C(in(v[it], (w / NN) % NSTATE));
     }
 #pragma unroll
     for (int it = 0; it < JL; ++it) {
-      const int w = tid + it * NT;
-      if (w < totj) {
-        const int e = w / (3 * NSTATE * NF);
-        const int r = w - e * (3 * NSTATE * NF);
-        sf[e][r / (NSTATE * NF)][NSTATE + (r / NF) % NSTATE][r % NF] = C(in(vj[it], (w / NF) % NSTATE));
+      const unsigned w = tid + it * NT;
+      if (w < (unsigned)totj) {
+        const unsigned ed = w / (NSTATE * NF), r = w - ed * (NSTATE * NF);   // ed = e * 3 + d
+        const unsigned s = r / NF;
+        // sf[e][d][NSTATE + s][ab]: (e * 3 + d) NFF NF + (NSTATE + s) NF + ab
+        (&sf[0][0][0][0])[ed * (NFF * NF) + NSTATE * NF + r] = C(in(vj[it], s));
       }
     }
   }
   __syncthreads();
 
-  for (int w = tid; w < ne * NN; w += NT) {
-    const int e = w / NN, node = w - e * NN;
-    const int ix = node % N, iy = (node / N) % N, iz = node / NF;
+  // one pass when a block's nodes fit the workgroup (every order's default
+  // tiling), a loop otherwise
+  constexpr unsigned GSTEP = E * NN <= NT ? 0x40000000u : NT;
+  for (unsigned w = tid; w < (unsigned)ne * NN; w += GSTEP) {
+    const unsigned e = w / NN, node = w - e * NN;
+    const unsigned iz = node / NF, rem = node - iz * NF;
+    const unsigned iy = rem / N, ix = rem - iy * N;
     C rx[N + 1], ry[N + 1], rz[N + 1];
 #pragma unroll
     for (int j = 0; j <= N; ++j) { rx[j] = sD[ix][j]; ry[j] = sD[iy][j]; rz[j] = sD[iz][j]; }
@@ -543,58 +549,76 @@ ldg_grad_kernel(const UT *__restrict__ U, TQ *__restrict__ Q, TQ *__restrict__ F
   // own trace is read from the volume state; both traces minus the
   // background in the FP64 layout, whose state is not stored as a
   // fluctuation -- exact, the values are close), then along b in place;
-  // face coordinates (a, b) = (y, z), (x, z), (x, y) for d = 0, 1, 2
+  // face coordinates (a, b) = (y, z), (x, z), (x, y) for d = 0, 1, 2.
+  // A line is (element, direction, field, position): w = fl N + x with
+  // fl = (e 3 + d) NFF + f, whose face slot starts at fl NF in sf; the two
+  // sweeps share the decomposition, done once per thread.
   {
-    constexpr bool FLUCT = sizeof(TQ) == 4;   // su holds fluctuations (mixed) or the state (FP64)
-    const C brho = FLUCT ? C(0) : C(d.ph.rho0), bE = FLUCT ? C(0) : C(d.ph.E0);
+    constexpr unsigned SW = 3 * NFF * N;              // lines per element
+    constexpr int SL = (E * SW + NT - 1) / NT;        // lines per thread
+    C *const sff = &sf[0][0][0][0];
+    unsigned lk[SL];                                  // fl << 3 | x, or ~0 past the block
+#pragma unroll
+    for (int it = 0; it < SL; ++it) {
+      const unsigned w = tid + it * NT, fl = w / N;
+      lk[it] = w < (unsigned)ne * SW ? fl << 3 | (w - fl * N) : ~0u;
+    }
     SymReg<N, C> I;
     I.load(sI);
-    for (int w = tid; w < ne * 3 * NFF * N; w += NT) {
-      const int e = w / (3 * NFF * N);
-      const int r = w - e * (3 * NFF * N);
-      const int dd = r / (NFF * N), f = (r / N) % NFF, b = r % N;
+    constexpr bool FLUCT = sizeof(TQ) == 4;   // su holds fluctuations (mixed) or the state (FP64)
+    const C brho = FLUCT ? C(0) : C(d.ph.rho0), bE = FLUCT ? C(0) : C(d.ph.E0);
+#pragma unroll
+    for (int it = 0; it < SL; ++it) {
+      if (lk[it] == ~0u) continue;
+      const unsigned fl = lk[it] >> 3, b = lk[it] & 7, f = fl % NFF;
       C v[N], o[N];
       if (f < NSTATE) {
+        // the face's nodes a + N b of direction dd: (0, a, b), (a, 0, b), (a, b, 0)
+        const unsigned ed = fl / NFF, e = ed / 3, dd = ed - 3 * e;
+        const unsigned as = dd == 0 ? N : 1, bs = dd == 2 ? N : NF;
+        const C *src = &su[e][f][bs * b];
         const C bg = f == 0 ? brho : f == 4 ? bE : C(0);
 #pragma unroll
-        for (int a = 0; a < N; ++a) v[a] = su[e][f][line_base<N>(dd, a, b)] - bg;
+        for (int a = 0; a < N; ++a) v[a] = src[as * a] - bg;
       } else {
         // the neighbour's trace is a state too (minus the background in the
         // FP64 layout), the gradient is not
         const C bg = f == NSTATE ? brho : f == 2 * NSTATE - 1 ? bE : C(0);
 #pragma unroll
-        for (int a = 0; a < N; ++a) v[a] = sf[e][dd][f][a + N * b] - bg;
+        for (int a = 0; a < N; ++a) v[a] = sff[fl * NF + a + N * b] - bg;
       }
       sym_apply<N, 1, C>(I.m, v, o);
 #pragma unroll
-      for (int q = 0; q < N; ++q) sf[e][dd][f][q + N * b] = o[q];
+      for (int q = 0; q < N; ++q) sff[fl * NF + q + N * b] = o[q];
     }
-  }
-  __syncthreads();
-  {
-    SymReg<N, C> I;
-    I.load(sI);
-    for (int w = tid; w < ne * 3 * NFF * N; w += NT) {
-      const int e = w / (3 * NFF * N);
-      const int r = w - e * (3 * NFF * N);
-      const int dd = r / (NFF * N), f = (r / N) % NFF, qa = r % N;
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < SL; ++it) {
+      if (lk[it] == ~0u) continue;
+      C *p = sff + (lk[it] >> 3) * NF + (lk[it] & 7);
       C v[N], o[N];
 #pragma unroll
-      for (int b = 0; b < N; ++b) v[b] = sf[e][dd][f][qa + N * b];
+      for (int b = 0; b < N; ++b) v[b] = p[N * b];
       sym_apply<N, 1, C>(I.m, v, o);
 #pragma unroll
-      for (int q = 0; q < N; ++q) sf[e][dd][f][qa + N * q] = o[q];
+      for (int q = 0; q < N; ++q) p[N * q] = o[q];
     }
   }
   __syncthreads();
 
-  // fluxes of both sides at every face Gauss point, weighted for the lift
+  // fluxes of both sides at every face Gauss point, weighted for the lift.
+  // The points of one direction are padded to whole waves, so that every
+  // wave evaluates one direction's flux (the direction is a template
+  // parameter: a wave holding points of several directions would run each
+  // variant with part of its lanes)
   {
     const Bg<C> bg(d.ph);
-    for (int w = tid; w < ne * 3 * NF; w += NT) {
-      const int e = w / (3 * NF);
-      const int r = w - e * (3 * NF);
-      const int dd = r / NF, ab = r % NF;
+    constexpr unsigned PD = (E * NF + 63) / 64 * 64;
+    constexpr unsigned FSTEP = 3 * PD <= NT ? 0x40000000u : NT;   // one pass when the points fit
+    for (unsigned w = tid; w < 3 * PD; w += FSTEP) {
+      const unsigned dd = w / PD, r = w - dd * PD;
+      if (r >= (unsigned)ne * NF) continue;
+      const unsigned e = r / NF, ab = r - e * NF;
       const C *fp = &sf[e][dd][0][ab];
       C wN[5], wK[5], q[NGRAD], outK[5], outN[5];
 #pragma unroll
@@ -692,17 +716,17 @@ __device__ inline void rhs_faces(const T *__restrict__ FF, const long (*snb)[6],
     T v[FL];
 #pragma unroll
     for (int it = 0; it < FL; ++it) {
-      const int w = tid + it * NT;
-      const int e = w / FE, r = w - e * FE;
-      const int dd = r / (2 * NS), side = (r / NS) & 1;
-      const long K = side ? snb[e < ne ? e : 0][2 * dd + 1] : k0 + e;
-      v[it] = e < ne ? FF[K * FE + r] : T(0);
+      const unsigned w = tid + it * NT;
+      const unsigned e = w / FE, r = w - e * FE;
+      const unsigned dd = r / (2 * NS), side = (r / NS) & 1;
+      const long K = side ? snb[e < (unsigned)ne ? e : 0][2 * dd + 1] : k0 + e;
+      v[it] = e < (unsigned)ne ? FF[K * FE + r] : T(0);
     }
 #pragma unroll
     for (int it = 0; it < FL; ++it) {
-      const int w = tid + it * NT;
-      const int e = w / FE, r = w - e * FE;
-      if (e < ne) sbuf[e][r] = v[it];
+      const unsigned w = tid + it * NT;
+      const unsigned e = w / FE, r = w - e * FE;
+      if (e < (unsigned)ne) sbuf[e][r] = v[it];
     }
   }
   __syncthreads();
@@ -710,8 +734,8 @@ __device__ inline void rhs_faces(const T *__restrict__ FF, const long (*snb)[6],
   {
     SymReg<N, T> It;
     It.load(so->It);
-    for (int w = tid; w < ne * (FE / N); w += NT) {
-      const int e = w / (FE / N), r = w - e * (FE / N);
+    for (unsigned w = tid; w < (unsigned)ne * (FE / N); w += NT) {
+      const unsigned e = w / (FE / N), r = w - e * (FE / N);
       T *p = &sbuf[e][N * r];
       T v[N], o[N];
 #pragma unroll
@@ -730,9 +754,9 @@ __device__ inline void rhs_faces(const T *__restrict__ FF, const long (*snb)[6],
     const int st_d = dd == 0 ? 1 : dd == 1 ? N : N * N;
     const int ast = dd == 0 ? N : 1;          // memory stride of the face coordinate a
     const int bst = dd == 2 ? N : N * N;      // memory stride of the face coordinate b
-    for (int w = tid; w < ne * 2 * NSTATE * N; w += NT) {
-      const int e = w / (2 * NSTATE * N), r = w - e * (2 * NSTATE * N);
-      const int side = r / (NSTATE * N), s = (r / N) % NSTATE, a = r % N;
+    for (unsigned w = tid; w < (unsigned)ne * 2 * NSTATE * N; w += NT) {
+      const unsigned e = w / (2 * NSTATE * N), r = w - e * (2 * NSTATE * N);
+      const unsigned side = r / (NSTATE * N), s = (r / N) % NSTATE, a = r % N;
       const T *p = &sbuf[e][(dd * 2 + side) * NS + s * NF + a];
       T v[N], o[N];
 #pragma unroll
