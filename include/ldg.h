@@ -99,11 +99,10 @@ int ldg_rhs(ldg_handle *h, const double *u, double *k, double *Du);
    (its rk4Rhs, solver.cpp:486-535, allocates scratch on every call).
    u and k are DEVICE arrays (dofs,5,K) in the layout above; the mesh,
    physics and precision come from cfg; there is no handle.  The operators,
-   instance tables, auxiliary gradient and face fluxes live in one fixed
-   device array of LDG_NOALLOC_SCRATCH_BYTES per GPU (held until the process
-   ends), so the mesh must satisfy
-   8192 + w K (15 (p+1)^3 + 30 (p+1)^2) <= LDG_NOALLOC_SCRATCH_BYTES with
-   w = 8 (LDG_FP64) or 4 (LDG_MIXED) bytes -- e.g. up to 14^3 elements at
+   instance tables and face fluxes live in one fixed device array of
+   LDG_NOALLOC_SCRATCH_BYTES per GPU (held until the process ends), so the
+   mesh must satisfy 8192 + 30 w K (p+1)^2 <= LDG_NOALLOC_SCRATCH_BYTES with
+   w = 8 (LDG_FP64) or 4 (LDG_MIXED) bytes -- e.g. up to 22^3 elements at
    p = 4 in FP64 -- and a larger one returns LDG_EINVAL.  Runs on `stream`
    (a hipStream_t; NULL: the default stream) and is complete when it
    returns; calls no hipMalloc/hipFree.  All calls on one GPU share its

@@ -525,7 +525,7 @@ int ldg_closure_collect(ldg_closure *c, ldg_handle *fine, ldg_handle *coarse, do
   // coarse right-hand side of the projected state, features and targets
   std::mt19937_64 rng(seed);
   for (Group &g : coarse->groups) {
-    if (launch_grad(coarse, g, state_in(g.u))) return cfail(c, LDG_EDEVICE, coarse->err);
+    if (launch_grad(coarse, g, state_in(g.u), -1, 1)) return cfail(c, LDG_EDEVICE, coarse->err);
     StageArgs st = empty_stage();
     st.kout = g.k;
     if (launch_rhs(coarse, g, state_in(g.u), st)) return cfail(c, LDG_EDEVICE, coarse->err);

@@ -121,7 +121,9 @@ struct StageIn {
 inline StageIn state_in(const double *u) { return StageIn{u, false}; }
 
 StageArgs empty_stage();
-int launch_grad(ldg_handle *h, Group &g, StageIn U, int c = -1);
+// wq: write the nodal gradient Q (1), not (0), or only when a closure is
+// attached (-1)
+int launch_grad(ldg_handle *h, Group &g, StageIn U, int c = -1, int wq = -1);
 int launch_rhs(ldg_handle *h, Group &g, StageIn U, const StageArgs &st, int c = -1);
 void drop_graph(ldg_handle *h);
 

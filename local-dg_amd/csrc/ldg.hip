@@ -95,22 +95,29 @@ Ops<N, double> make_ops(const Basis1D &B, double &asym) {
     o.L0[a] = B.at(B.M1inv, a, 0);
     o.w[a] = B.w[a];
   }
+  for (int q = 0; q < N; ++q) {
+    o.IL0[q] = 0.0;
+    for (int j = 0; j < N; ++j) o.IL0[q] += o.I[q][j] * o.L0[j];
+  }
   // sweep operators in even/odd split form, from their (anti-)centro-symmetric
   // parts (the node sets are symmetric; this removes construction round-off)
-  double I[N][N], It[N][N], Dt[N][N], Mi[N][N];
+  double I[N][N], It[N][N], Dt[N][N], Mi[N][N], D[N][N];
   for (int a = 0; a < N; ++a)
     for (int b = 0; b < N; ++b) {
       I[a][b] = o.I[a][b];
       It[a][b] = o.I[b][a];
       Dt[a][b] = o.D[b][a];
       Mi[a][b] = o.Minv[a][b];
+      D[a][b] = o.D[a][b];
     }
   asym = std::max(std::max(symmetrize_checked<N>(I, 1), symmetrize_checked<N>(It, 1)),
                   std::max(symmetrize_checked<N>(Dt, -1), symmetrize_checked<N>(Mi, 1)));
+  asym = std::max(asym, symmetrize_checked<N>(D, -1));
   make_symmat<N, double>(I, o.sI);
   make_symmat<N, double>(It, o.sIt);
   make_symmat<N, double>(Dt, o.sDt);
   make_symmat<N, double>(Mi, o.sMi);
+  make_symmat<N, double>(D, o.sD);
   return o;
 }
 
@@ -142,19 +149,32 @@ StageArgs empty_stage() {
 // ---------------------------------------------------------------- launches
 
 template <int N, int E>
-void launch_grad_e(ldg_handle *h, Group &g, const Tiling &t, StageIn U) {
+void launch_grad_e(ldg_handle *h, Group &g, const Tiling &t, StageIn U, bool wq) {
   LaunchArgs la{t.d_inst, t.d_blk, t.nblocks, (t.nblocks + 7) / 8};
   const Ops<N, double> *op = (const Ops<N, double> *)g.ops;
   const dim3 grid(8 * la.per_xcd), block(NT);
-  if (h->mixed && U.fluct)
-    hipLaunchKernelGGL((ldg_grad_kernel<N, E, float, float>), grid, block, 0, h->stream, (const float *)U.p,
-                       (float *)g.q, (float *)g.ff, la, op);
-  else if (h->mixed)
-    hipLaunchKernelGGL((ldg_grad_kernel<N, E, float, double>), grid, block, 0, h->stream, (const double *)U.p,
-                       (float *)g.q, (float *)g.ff, la, op);
-  else
-    hipLaunchKernelGGL((ldg_grad_kernel<N, E, double, double>), grid, block, 0, h->stream, (const double *)U.p,
-                       (double *)g.q, (double *)g.ff, la, op);
+  if (h->mixed && U.fluct) {
+    if (wq)
+      hipLaunchKernelGGL((ldg_grad_kernel<N, E, float, float, true>), grid, block, 0, h->stream, (const float *)U.p,
+                         (float *)g.q, (float *)g.ff, la, op);
+    else
+      hipLaunchKernelGGL((ldg_grad_kernel<N, E, float, float, false>), grid, block, 0, h->stream, (const float *)U.p,
+                         (float *)g.q, (float *)g.ff, la, op);
+  } else if (h->mixed) {
+    if (wq)
+      hipLaunchKernelGGL((ldg_grad_kernel<N, E, float, double, true>), grid, block, 0, h->stream, (const double *)U.p,
+                         (float *)g.q, (float *)g.ff, la, op);
+    else
+      hipLaunchKernelGGL((ldg_grad_kernel<N, E, float, double, false>), grid, block, 0, h->stream,
+                         (const double *)U.p, (float *)g.q, (float *)g.ff, la, op);
+  } else {
+    if (wq)
+      hipLaunchKernelGGL((ldg_grad_kernel<N, E, double, double, true>), grid, block, 0, h->stream,
+                         (const double *)U.p, (double *)g.q, (double *)g.ff, la, op);
+    else
+      hipLaunchKernelGGL((ldg_grad_kernel<N, E, double, double, false>), grid, block, 0, h->stream,
+                         (const double *)U.p, (double *)g.q, (double *)g.ff, la, op);
+  }
 }
 
 template <int N, int E>
@@ -164,19 +184,19 @@ void launch_rhs_e(ldg_handle *h, Group &g, const Tiling &t, StageIn U, const Sta
   const dim3 grid(8 * la.per_xcd), block(NT);
   if (h->mixed && U.fluct)
     hipLaunchKernelGGL((ldg_rhs_kernel<N, E, float, float>), grid, block, 0, h->stream, (const float *)U.p,
-                       (const float *)g.q, (const float *)g.ff, la, op, st);
+                       (const float *)g.ff, la, op, st);
   else if (h->mixed)
     hipLaunchKernelGGL((ldg_rhs_kernel<N, E, float, double>), grid, block, 0, h->stream, (const double *)U.p,
-                       (const float *)g.q, (const float *)g.ff, la, op, st);
+                       (const float *)g.ff, la, op, st);
   else
     hipLaunchKernelGGL((ldg_rhs_kernel<N, E, double, double>), grid, block, 0, h->stream, (const double *)U.p,
-                       (const double *)g.q, (const double *)g.ff, la, op, st);
+                       (const double *)g.ff, la, op, st);
 }
 
 template <int N>
-void launch_grad_n(ldg_handle *h, Group &g, int c, StageIn U) {
-  if (c == 0) launch_grad_e<N, Cand<N>::grad[0]>(h, g, g.grad[0], U);
-  else launch_grad_e<N, Cand<N>::grad[1]>(h, g, g.grad[1], U);
+void launch_grad_n(ldg_handle *h, Group &g, int c, StageIn U, bool wq) {
+  if (c == 0) launch_grad_e<N, Cand<N>::grad[0]>(h, g, g.grad[0], U, wq);
+  else launch_grad_e<N, Cand<N>::grad[1]>(h, g, g.grad[1], U, wq);
 }
 
 template <int N>
@@ -185,15 +205,18 @@ void launch_rhs_n(ldg_handle *h, Group &g, int c, StageIn U, const StageArgs &st
   else launch_rhs_e<N, Cand<N>::rhs[1]>(h, g, g.rhs[1], U, st);
 }
 
-int launch_grad(ldg_handle *h, Group &g, StageIn U, int c) {
+int launch_grad(ldg_handle *h, Group &g, StageIn U, int c, int wq) {
   if (U.fluct && !h->mixed) return fail(h, LDG_EINVAL, "FP32 stage input outside the mixed path");
   if (c < 0) c = g.cgrad;
+  // the nodal gradient is written when asked for, and whenever a closure
+  // (whose features read it) is attached
+  const bool w = wq > 0 || (wq < 0 && g.cl.net != nullptr);
   switch (g.n) {
-    case 2: launch_grad_n<2>(h, g, c, U); break;
-    case 3: launch_grad_n<3>(h, g, c, U); break;
-    case 4: launch_grad_n<4>(h, g, c, U); break;
-    case 5: launch_grad_n<5>(h, g, c, U); break;
-    case 6: launch_grad_n<6>(h, g, c, U); break;
+    case 2: launch_grad_n<2>(h, g, c, U, w); break;
+    case 3: launch_grad_n<3>(h, g, c, U, w); break;
+    case 4: launch_grad_n<4>(h, g, c, U, w); break;
+    case 5: launch_grad_n<5>(h, g, c, U, w); break;
+    case 6: launch_grad_n<6>(h, g, c, U, w); break;
     default: return fail(h, LDG_EINVAL, "unsupported order");
   }
   HIPCHECK(h, hipGetLastError());
@@ -840,7 +863,7 @@ int ldg_rhs(ldg_handle *h, const double *u, double *k, double *Du) {
   const size_t bytes = g.nU * sizeof(double);
   // the FP64 running-sum array is free between steps: it takes the input
   HIPCHECK(h, hipMemcpyAsync(g.acc, u, bytes, hipMemcpyHostToDevice, h->stream));
-  int rc = launch_grad(h, g, state_in(g.acc));
+  int rc = launch_grad(h, g, state_in(g.acc), -1, Du ? 1 : -1);
   if (rc) return rc;
   StageArgs st = empty_stage();
   st.kout = g.k;
@@ -1061,14 +1084,14 @@ int ldg_kernel_times(ldg_handle *h, double dt, int nsteps, double ms[2]) {
 //
 // ldg_rhs_noalloc keeps everything beyond its input and output in one fixed
 // device array per GPU: the 1D operators, the two instance tables (gradient
-// and right-hand-side tilings), the auxiliary gradient Q and the face fluxes.  Host code never
+// and right-hand-side tilings) and the face fluxes.  Host code never
 // uses the array's name as an address: its device address is looked up once
 // per device with hipGetSymbolAddress and kept in g_scratch_ptr.
 static __device__ __align__(256) unsigned char scratch[LDG_NOALLOC_SCRATCH_BYTES];
 
 namespace ldg_host {
 
-constexpr size_t kScrOps = 0, kScrInstGrad = 4096, kScrInstRhs = 6144, kScrQ = 8192;
+constexpr size_t kScrOps = 0, kScrInstGrad = 4096, kScrInstRhs = 6144, kScrFF = 8192;
 static_assert(sizeof(Ops<6, double>) <= kScrInstGrad - kScrOps, "operators do not fit their scratch slot");
 static_assert(sizeof(InstDesc) <= kScrInstRhs - kScrInstGrad, "instance table does not fit its scratch slot");
 
@@ -1149,19 +1172,20 @@ hipError_t noalloc_launch(const ldg_config &cfg, const Inst &in, unsigned char *
                       (dsc[1].nblocks + 7) / 8};
   StageArgs st = empty_stage();
   st.kout = k;
-  const size_t nq = (size_t)in.K * NGRAD * N * N * N;   // gradient values; the face fluxes follow
+  // the gradient kernel writes no nodal gradient here (WQ = false): the
+  // scratch holds the face fluxes only
   if (cfg.precision == LDG_MIXED) {
-    float *q = reinterpret_cast<float *>(scr + kScrQ), *ff = q + nq;
-    hipLaunchKernelGGL((ldg_grad_kernel<N, EG, float, double>), dim3(8 * lg.per_xcd), dim3(NT), 0, s, u, q, ff, lg,
-                       op);
+    float *ff = reinterpret_cast<float *>(scr + kScrFF);
+    hipLaunchKernelGGL((ldg_grad_kernel<N, EG, float, double, false>), dim3(8 * lg.per_xcd), dim3(NT), 0, s, u,
+                       nullptr, ff, lg, op);
     hipLaunchKernelGGL((ldg_rhs_kernel<N, ER, float, double>), dim3(8 * lr.per_xcd), dim3(NT), 0, s, u,
-                       (const float *)q, (const float *)ff, lr, op, st);
+                       (const float *)ff, lr, op, st);
   } else {
-    double *q = reinterpret_cast<double *>(scr + kScrQ), *ff = q + nq;
-    hipLaunchKernelGGL((ldg_grad_kernel<N, EG, double, double>), dim3(8 * lg.per_xcd), dim3(NT), 0, s, u, q, ff, lg,
-                       op);
+    double *ff = reinterpret_cast<double *>(scr + kScrFF);
+    hipLaunchKernelGGL((ldg_grad_kernel<N, EG, double, double, false>), dim3(8 * lg.per_xcd), dim3(NT), 0, s, u,
+                       nullptr, ff, lg, op);
     hipLaunchKernelGGL((ldg_rhs_kernel<N, ER, double, double>), dim3(8 * lr.per_xcd), dim3(NT), 0, s, u,
-                       (const double *)q, (const double *)ff, lr, op, st);
+                       (const double *)ff, lr, op, st);
   }
   return hipGetLastError();
 }
@@ -1190,13 +1214,13 @@ extern "C" int ldg_rhs_noalloc(const ldg_config *cfg, const double *u, double *k
   in.group = 0;
   in.slot = 0;
   const int n = in.p + 1;
-  // the gradient (15 values per node) and the face fluxes (30 per face point)
-  const double qbytes = (double)in.K * (NGRAD * n * n * n + 30 * n * n) * (cfg->precision == LDG_MIXED ? 4.0 : 8.0);
-  if (kScrQ + qbytes > (double)LDG_NOALLOC_SCRATCH_BYTES) {
+  // the face fluxes (30 values per face point of an element)
+  const double qbytes = (double)in.K * (30 * n * n) * (cfg->precision == LDG_MIXED ? 4.0 : 8.0);
+  if (kScrFF + qbytes > (double)LDG_NOALLOC_SCRATCH_BYTES) {
     char msg[256];
     std::snprintf(msg, sizeof(msg),
                   "ldg_rhs_noalloc: %d x %d x %d elements at p = %d need %.0f bytes of device scratch; the limit is "
-                  "%u (LDG_NOALLOC_SCRATCH_BYTES)", in.nx, in.ny, in.nz, in.p, kScrQ + qbytes,
+                  "%u (LDG_NOALLOC_SCRATCH_BYTES)", in.nx, in.ny, in.nz, in.p, kScrFF + qbytes,
                   (unsigned)LDG_NOALLOC_SCRATCH_BYTES);
     return noalloc_fail(LDG_EINVAL, msg);
   }

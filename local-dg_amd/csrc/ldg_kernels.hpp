@@ -2,8 +2,11 @@
 //
 // One RK4 stage is two launches over the whole periodic mesh:
 //   ldg_grad_kernel  auxiliary gradient Q_l = (1/alpha_l)(Dn U + L0 jump_l)
-//                    (Solver::rk4Rhs first half, solver.cpp:490-508)
-//   ldg_rhs_kernel   face Roe + one-sided viscous fluxes, volume integrals,
+//                    (Solver::rk4Rhs first half, solver.cpp:490-508) on the
+//                    faces, and the Roe + one-sided viscous face fluxes of
+//                    both elements of every face (solver.cpp:643-698,
+//                    750-814, 891-921, 1194-1374)
+//   ldg_rhs_kernel   face lifts, the volume gradient and volume integrals,
 //                    inverse mass and the fused RK4 stage epilogue
 //                    (solver.cpp:510-535 and 410-434)
 // Both stage a block of E consecutive elements in LDS, work in 1D "lines"
@@ -26,7 +29,10 @@
 //   u0, acc, unext : FP64 (node, state, element) -- the reference layout
 //   U (stage input): the same, or in the mixed path (RK stages 2-4) FP32
 //                    fluctuations about the background in that layout
-//   Q              : TQ (node, s + 5*l, element)  -- 15 gradient fields per element
+//   FF             : T, per element [direction][side][state][face point]:
+//                    the weighted face fluxes (gradient kernel -> rhs kernel)
+//   Q              : TQ (node, s + 5*l, element) -- the nodal gradient, 15
+//                    fields per element, written only when asked for
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -47,9 +53,10 @@ struct Ops {
   T Dn[N][N];     // Dn[i][j] = l_j'(r_i)
   T Minv[N][N];   // inverse 1D mass matrix
   T L0[N];        // Minv[:, 0]: lift of the jump at the left face
+  T IL0[N];       // I L0: that lift at the Gauss points
   T w[N];         // Gauss weights
-  // the sweep operators in even/odd split form: I, I^T, D^T, Minv
-  SymMat<N, T> sI, sIt, sDt, sMi;
+  // the sweep operators in even/odd split form: I, I^T, D^T, Minv, D
+  SymMat<N, T> sI, sIt, sDt, sMi, sD;
 };
 
 struct MeshArgs {
@@ -367,8 +374,8 @@ __device__ inline double block_sum(double v, double *sw) {
 // form: -M1^{-1} S1 + face lifts = Dn + M1^{-1} e_0 e_0^T, exactly), and
 // the face fluxes of the elements' negative faces.
 //
-// Gradient: computed in the gradient's own type TQ: FP64, or FP32 in the
-// mixed path (operator entries and the FP32 input fluctuations in FP32 LDS).
+// Gradient: computed in the gradient's own type TQ (FP64, or FP32 in the
+// mixed path) from the fluctuation of the state about the background.
 // One thread per node computes all 15 components and stores them straight
 // to HBM (consecutive threads -> consecutive nodes, coalesced).
 //
@@ -376,13 +383,29 @@ __device__ inline double block_sum(double v, double *sw) {
 // K's trace (already staged here for the gradient's jump term), N's trace
 // and N's gradient trace (the one-sided switch picks it for both sides), so
 // the workgroup that owns N evaluates that face once for both elements:
-// interpolation of the 25 trace fields to the face Gauss points (two
-// sweeps), then Roe + viscous fluxes (face_flux_pair), weighted for the lift
+// interpolation of the two states and the 11 gradient components the
+// viscous flux reads to the face Gauss points (two sweeps), then Roe +
+// viscous fluxes (face_flux_pair), weighted for the lift
 // (w_a w_b 2/dx_d), written to FF[element][d][side][state][face point] with
 // side 0 = N's own flux on its -d face and side 1 = K's flux on its +d face.
 // The right-hand-side kernel only lifts them.  In type TQ, on fluctuations
 // about the background, like the rest of the flux arithmetic.
 // ---------------------------------------------------------------------
+// Gradient components that the viscous flux across a face of direction D
+// reads (numericalFluxV, solver.cpp:891-921, with the face normal +-e_D:
+// the stress row sigma_D. and the heat flux along D): the derivatives of all
+// five conserved variables along D, and along each other direction l those
+// of rho, m_D and m_l -- 11 of the 15.  The other four enter the
+// reference's face flux multiplied by a zero normal component, so only
+// these are carried to the face points.  Slot j of direction D holds
+// component k = s + 5 l.
+constexpr int NFG = 11;
+__host__ __device__ constexpr int face_grad(int D, int j) {
+  return j < NSTATE ? j + NSTATE * D
+                    : ((j - NSTATE) % 3 == 0 ? 0 : (j - NSTATE) % 3 == 1 ? 1 + D : 1 + (D + 1 + (j - NSTATE) / 3) % 3) +
+                          NSTATE * ((D + 1 + (j - NSTATE) / 3) % 3);
+}
+
 // Occupancy target of the gradient kernel (waves per SIMD), which sets its
 // register budget: 5 with an FP32 gradient up to p = 4 (the face flux needs
 // more registers than the 80 of 6 waves), 4 with FP64 and at p = 5; never
@@ -390,7 +413,7 @@ __device__ inline double block_sum(double v, double *sw) {
 // 160 KiB of LDS per CU).
 template <int N, int E, typename TQ>
 struct GradWaves {
-  static constexpr int NFF = 2 * NSTATE + NGRAD;
+  static constexpr int NFF = 2 * NSTATE + NFG;
   static constexpr int lds = (int)sizeof(TQ) * (E * NSTATE * N * N * N + E * 3 * NFF * N * N + N * (N + 1) + N) +
                              (int)sizeof(SymMat<N, TQ>) + 8 * 6 * E;
   static constexpr int by_lds = (160 * 1024) / lds;
@@ -403,12 +426,77 @@ struct GradWaves {
 template <int N>
 __host__ __device__ constexpr int ff_per_element() { return 3 * 2 * NSTATE * N * N; }
 
-template <int N, int E, typename TQ, typename UT>
+// Face fluxes of direction D at one face point: fp points at the point's
+// slot of field 0 (fields NF apart: own trace 0-4, neighbour's trace 5-9,
+// the gradient components face_grad(D, j) 10-20).
+template <typename C, int D, int NF>
+__device__ inline void face_point_flux(const C *fp, const Bg<C> &bg, C outK[5], C outN[5]) {
+  C wN[5], wK[5], q[NGRAD];
+#pragma unroll
+  for (int s = 0; s < NSTATE; ++s) { wN[s] = fp[s * NF]; wK[s] = fp[(NSTATE + s) * NF]; }
+#pragma unroll
+  for (int k = 0; k < NGRAD; ++k) q[k] = C(0);   // not read by the direction-D flux
+#pragma unroll
+  for (int j = 0; j < NFG; ++j) q[face_grad(D, j)] = fp[(2 * NSTATE + j) * NF];
+  face_flux_pair<C, D>(wK, wN, q, bg, outK, outN);
+}
+
+// Gradient components of one -D face node for the face flux (the sums of
+// the full gradient at that node): along D the line through the node,
+// along the two other directions l lines in the face plane, each with the
+// lift of the jump on its -l face (the neighbour traces staged in sf); the
+// node's own trace with them.  su, sf: the element's state and face slots.
+template <int N, int D, int NFF, typename C>
+__device__ inline void face_node_grad(const C *__restrict__ su, C *__restrict__ sf, const C (*__restrict__ sD)[N + 1],
+                                      const double *ia, unsigned ab) {
+  constexpr int NN = N * N * N, NF = N * N;
+  constexpr unsigned ST[3] = {1, N, NF};   // memory stride along x, y, z
+  const unsigned a = ab % N, b = ab / N;
+  // the node (0, a, b), (a, 0, b) or (a, b, 0)
+  const unsigned c[3] = {D == 0 ? 0u : a, D == 0 ? a : D == 1 ? 0u : b, D == 2 ? 0u : b};
+  const unsigned node = c[0] + N * c[1] + NF * c[2];
+  C *fs = sf + D * NFF * NF + ab;
+#pragma unroll
+  for (int s = 0; s < NSTATE; ++s) {
+    const C *us = su + s * NN + node;
+    C q = sD[0][N] * sf[(D * NFF + NSTATE + s) * NF + ab];
+#pragma unroll
+    for (int j = 0; j < N; ++j) q += sD[0][j] * us[j * ST[D]];
+    fs[s * NF] = us[0];
+    fs[(2 * NSTATE + s) * NF] = q * C(ia[D]);
+  }
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    const int l = (D + 1 + t) % 3;
+    const unsigned il = c[l];
+    // the node's coordinates on the -l face
+    const unsigned abl = l == 0 ? c[1] + N * c[2] : l == 1 ? c[0] + N * c[2] : c[0] + N * c[1];
+    const unsigned start = node - il * ST[l];
+    const C *row = sD[il];
+#pragma unroll
+    for (int idx = 0; idx < 3; ++idx) {
+      const int s = idx == 0 ? 0 : idx == 1 ? 1 + D : 1 + l;
+      const C *us = su + s * NN + start;
+      C q = row[N] * sf[(l * NFF + NSTATE + s) * NF + abl];
+#pragma unroll
+      for (int j = 0; j < N; ++j) q += row[j] * us[j * ST[l]];
+      fs[(2 * NSTATE + NSTATE + 3 * t + idx) * NF] = q * C(ia[l]);
+    }
+  }
+}
+
+// WQ: also compute the gradient at every node and write it to Q (for the
+// closure's features and ldg_rhs's gradient output); the time-stepping path
+// needs only the face nodes' components, the right-hand-side kernel
+// computes the volume gradient at the Gauss points itself.
+template <int N, int E, typename TQ, typename UT, bool WQ>
 __global__ void __launch_bounds__(NT, (GradWaves<N, E, TQ>::value))
 ldg_grad_kernel(const UT *__restrict__ U, TQ *__restrict__ Q, TQ *__restrict__ FF, LaunchArgs la,
                 const Ops<N, double> *__restrict__ opp) {
   constexpr int NN = N * N * N, NF = N * N;
-  constexpr int NFF = 2 * NSTATE + NGRAD;   // face fields: own U (0-4), the -d neighbour's U (5-9), own Q (10-24)
+  // face fields of each element's -d face: own U (0-4), the -d neighbour's U
+  // (5-9), the gradient components of direction d (10-20)
+  constexpr int NFF = 2 * NSTATE + NFG;
   // arithmetic type: FP32 with an FP32 gradient (mixed), FP64 otherwise
   using C = TQ;
   __shared__ C su[E][NSTATE][NN];
@@ -425,7 +513,7 @@ ldg_grad_kernel(const UT *__restrict__ U, TQ *__restrict__ Q, TQ *__restrict__ F
   const InstDesc &d = block_instance<E>(la, lb, m, ec, kl, ne);
   if (ne == 0) return;                // padding block of a partial brick
   const long k0 = d.elem_off + kl;    // first element of the block in the arrays
-  const int tid = threadIdx.x;
+  const unsigned tid = threadIdx.x;
   fill_neighbors(m, ec, d.elem_off, ne, snb, tid);
 
   // Q_l(i) = (Dn u)(i) + L0(i) (u_0 - u_nbr): the own trace u_0 is the line's
@@ -439,68 +527,73 @@ ldg_grad_kernel(const UT *__restrict__ U, TQ *__restrict__ Q, TQ *__restrict__ F
   if (tid >= 128 && tid < 128 + N) sw[tid - 128] = C(opp->w[tid - 128]);
   __syncthreads();   // neighbour table
 
-  // the block's state (contiguous) and the -d neighbours' traces on their
-  // positive faces into LDS, all of a thread's loads issued before the first
-  // is stored.  The loads are unconditional (an index past the block is
-  // clamped to its last word) and converted only when stored: a load under
-  // a branch is waited for at the branch's end.
+  // the block's state (one node per thread, its five variables) and the -d
+  // neighbours' traces on their positive faces (one face point per thread)
+  // into LDS as fluctuations about the background, all of a thread's loads
+  // issued before the first is stored.  The loads are unconditional (an
+  // index past the block is clamped to its last item) and converted only
+  // when stored: a load under a branch is waited for at the branch's end.
+  // An FP64 input in the mixed path (RK stage 1) is rounded to the FP32
+  // fluctuation the other stages read, so a step's result does not depend
+  // on which form its input came in; the FP64 path differentiates the
+  // fluctuation too (the same gradient: Dn annihilates constants).
   {
-    constexpr int UL = (E * NSTATE * NN + NT - 1) / NT;
-    constexpr int JL = (E * 3 * NSTATE * NF + NT - 1) / NT;
-    const UT *Ub = U + k0 * NSTATE * NN;
-    const int tot = ne * NSTATE * NN;
-    // trace item w = ((e * 3 + d) * NSTATE + s) * NF + ab
-    const int totj = ne * 3 * NSTATE * NF;
-    UT v[UL], vj[JL];
+    constexpr int UL = (E * NN + NT - 1) / NT;
+    constexpr int JL = (E * 3 * NF + NT - 1) / NT;
+    const unsigned tot = ne * NN, totj = ne * 3 * NF;
+    UT v[UL][NSTATE], vj[JL][NSTATE];
+    unsigned so[UL], sj[JL];
 #pragma unroll
     for (int it = 0; it < UL; ++it) {
-      const int w = tid + it * NT;
-      v[it] = Ub[w < tot ? w : tot - 1];
+      const unsigned w = tid + it * NT < tot ? tid + it * NT : tot - 1;
+      const unsigned e = w / NN;
+      so[it] = w + e * ((NSTATE - 1) * NN);   // su[e][0][node]
+      const UT *src = U + (k0 + e) * (NSTATE * NN) + (w - e * NN);
+#pragma unroll
+      for (int s = 0; s < NSTATE; ++s) v[it][s] = src[s * NN];
     }
 #pragma unroll
     for (int it = 0; it < JL; ++it) {
-      // unsigned: a division by a constant is a multiply-high and a shift
       const unsigned w = tid + it * NT < totj ? tid + it * NT : totj - 1;
-      const unsigned ed = w / (NSTATE * NF), r = w - ed * (NSTATE * NF);   // ed = e * 3 + d
-      const unsigned e = ed / 3, d = ed - 3 * e;
-      const unsigned s = r / NF, ab = r - s * NF;
+      const unsigned ed = w / NF, ab = w - ed * NF;   // ed = e * 3 + d
+      const unsigned e = ed / 3, dd = ed - 3 * e;
       const unsigned b = ab / N, a = ab - b * N;
+      sj[it] = ed * (NFF * NF) + NSTATE * NF + ab;    // sf[e][dd][NSTATE][ab]
       // node (N-1, a, b), (a, N-1, b) or (a, b, N-1) of the -d neighbour: its +d face
-      const unsigned node = (d == 0 ? N : 1) * a + (d == 2 ? N : NF) * b + (d == 0 ? 1 : d == 1 ? N : NF) * (N - 1);
-      vj[it] = U[snb[e][2 * d] * (NSTATE * NN) + (s * NN + node)];
+      const unsigned node = (dd == 0 ? N : 1) * a + (dd == 2 ? N : NF) * b + (dd == 0 ? 1 : dd == 1 ? N : NF) * (N - 1);
+      const UT *src = U + snb[e][2 * dd] * (NSTATE * NN) + node;
+#pragma unroll
+      for (int s = 0; s < NSTATE; ++s) vj[it][s] = src[s * NN];
     }
-    // an FP64 input in the mixed path (RK stage 1) is rounded to the FP32
-    // fluctuation the other stages read, so a step's result does not depend
-    // on which form its input came in
-    constexpr bool ROUND = sizeof(TQ) == 4 && sizeof(UT) == 8;
-    const double brho = ROUND ? d.ph.rho0 : 0.0, bE = ROUND ? d.ph.E0 : 0.0;
-    auto in = [&](UT x, int s) -> double {
-      if (!ROUND) return (double)x;
-      const double b = s == 0 ? brho : s == 4 ? bE : 0.0;
-      return (double)(float)((double)x - b);
+    const double brho = d.ph.rho0, bE = d.ph.E0;
+    auto fluct = [&](UT x, int s) -> C {
+      if (sizeof(UT) == 4) return C(x);   // already the FP32 fluctuation
+      const double f = (double)x - (s == 0 ? brho : s == 4 ? bE : 0.0);
+      return C(f);
     };
+    C *const suf = &su[0][0][0];
+    C *const sff = &sf[0][0][0][0];
 #pragma unroll
     for (int it = 0; it < UL; ++it) {
-      const unsigned w = tid + it * NT;
-      if (w < (unsigned)tot) (&su[0][0][0])[w] = C(in(v[it], (w / NN) % NSTATE));
+      if (tid + it * NT < tot) {
+#pragma unroll
+        for (int s = 0; s < NSTATE; ++s) suf[so[it] + s * NN] = fluct(v[it][s], s);
+      }
     }
 #pragma unroll
     for (int it = 0; it < JL; ++it) {
-      const unsigned w = tid + it * NT;
-      if (w < (unsigned)totj) {
-        const unsigned ed = w / (NSTATE * NF), r = w - ed * (NSTATE * NF);   // ed = e * 3 + d
-        const unsigned s = r / NF;
-        // sf[e][d][NSTATE + s][ab]: (e * 3 + d) NFF NF + (NSTATE + s) NF + ab
-        (&sf[0][0][0][0])[ed * (NFF * NF) + NSTATE * NF + r] = C(in(vj[it], s));
+      if (tid + it * NT < totj) {
+#pragma unroll
+        for (int s = 0; s < NSTATE; ++s) sff[sj[it] + s * NF] = fluct(vj[it][s], s);
       }
     }
   }
   __syncthreads();
 
-  // one pass when a block's nodes fit the workgroup (every order's default
-  // tiling), a loop otherwise
-  constexpr unsigned GSTEP = E * NN <= NT ? 0x40000000u : NT;
-  for (unsigned w = tid; w < (unsigned)ne * NN; w += GSTEP) {
+  // gradient, one node per thread; the node's own trace on the element's -d
+  // faces and the face components of its gradient go to the face slots (a
+  // node on an edge or corner belongs to several faces)
+  auto node_grad = [&](unsigned w) {
     const unsigned e = w / NN, node = w - e * NN;
     const unsigned iz = node / NF, rem = node - iz * NF;
     const unsigned iy = rem / N, ix = rem - iy * N;
@@ -508,7 +601,7 @@ ldg_grad_kernel(const UT *__restrict__ U, TQ *__restrict__ Q, TQ *__restrict__ F
 #pragma unroll
     for (int j = 0; j <= N; ++j) { rx[j] = sD[ix][j]; ry[j] = sD[iy][j]; rz[j] = sD[iz][j]; }
     TQ *Qe = Q + (k0 + e) * NGRAD * NN + node;
-    C g[NGRAD];
+    C g[NGRAD], ox[NSTATE], oy[NSTATE], oz[NSTATE];
 #pragma unroll
     for (int s = 0; s < NSTATE; ++s) {
       const C *us = su[e][s];
@@ -517,9 +610,11 @@ ldg_grad_kernel(const UT *__restrict__ U, TQ *__restrict__ Q, TQ *__restrict__ F
       C qz = rz[N] * sf[e][2][NSTATE + s][ix + N * iy];
 #pragma unroll
       for (int j = 0; j < N; ++j) {
-        qx += rx[j] * us[j + N * iy + NF * iz];
-        qy += ry[j] * us[ix + N * j + NF * iz];
-        qz += rz[j] * us[ix + N * iy + NF * j];
+        const C ux = us[j + N * iy + NF * iz], uy = us[ix + N * j + NF * iz], uz = us[ix + N * iy + NF * j];
+        if (j == 0) { ox[s] = ux; oy[s] = uy; oz[s] = uz; }   // the line's first node: the -d face trace
+        qx += rx[j] * ux;
+        qy += ry[j] * uy;
+        qz += rz[j] * uz;
       }
       g[0 * NSTATE + s] = qx * C(m.ialpha[0]);
       g[1 * NSTATE + s] = qy * C(m.ialpha[1]);
@@ -528,80 +623,89 @@ ldg_grad_kernel(const UT *__restrict__ U, TQ *__restrict__ Q, TQ *__restrict__ F
       Qe[(1 * NSTATE + s) * NN] = g[1 * NSTATE + s];
       Qe[(2 * NSTATE + s) * NN] = g[2 * NSTATE + s];
     }
-    // the gradient on the element's -d faces, for their fluxes (a node on an
-    // edge or corner belongs to several)
     if (ix == 0) {
+      C *fs = &sf[e][0][0][iy + N * iz];
 #pragma unroll
-      for (int k = 0; k < NGRAD; ++k) sf[e][0][2 * NSTATE + k][iy + N * iz] = g[k];
+      for (int s = 0; s < NSTATE; ++s) fs[s * NF] = ox[s];
+#pragma unroll
+      for (int j = 0; j < NFG; ++j) fs[(2 * NSTATE + j) * NF] = g[face_grad(0, j)];
     }
     if (iy == 0) {
+      C *fs = &sf[e][1][0][ix + N * iz];
 #pragma unroll
-      for (int k = 0; k < NGRAD; ++k) sf[e][1][2 * NSTATE + k][ix + N * iz] = g[k];
+      for (int s = 0; s < NSTATE; ++s) fs[s * NF] = oy[s];
+#pragma unroll
+      for (int j = 0; j < NFG; ++j) fs[(2 * NSTATE + j) * NF] = g[face_grad(1, j)];
     }
     if (iz == 0) {
+      C *fs = &sf[e][2][0][ix + N * iy];
 #pragma unroll
-      for (int k = 0; k < NGRAD; ++k) sf[e][2][2 * NSTATE + k][ix + N * iy] = g[k];
+      for (int s = 0; s < NSTATE; ++s) fs[s * NF] = oz[s];
+#pragma unroll
+      for (int j = 0; j < NFG; ++j) fs[(2 * NSTATE + j) * NF] = g[face_grad(2, j)];
+    }
+  };
+  if constexpr (WQ) {
+    if constexpr (E * NN <= NT) {
+      if (tid < (unsigned)ne * NN) node_grad(tid);   // one pass (every order's default tiling)
+    } else {
+      for (unsigned w = tid; w < (unsigned)ne * NN; w += NT) node_grad(w);
+    }
+  } else {
+    // the face nodes only; the points of one direction padded to whole
+    // waves, as in the flux phase below
+    constexpr unsigned PD = (E * NF + 63) / 64 * 64;
+    constexpr unsigned FSTEP = 3 * PD <= NT ? 0x40000000u : NT;
+    for (unsigned w = tid; w < 3 * PD; w += FSTEP) {
+      const unsigned dd = w / PD, r = w - dd * PD;
+      if (r >= (unsigned)ne * NF) continue;
+      const unsigned e = r / NF, ab = r - e * NF;
+      if (dd == 0) face_node_grad<N, 0, NFF, C>(su[e][0], sf[e][0][0], sD, m.ialpha, ab);
+      else if (dd == 1) face_node_grad<N, 1, NFF, C>(su[e][0], sf[e][0][0], sD, m.ialpha, ab);
+      else face_node_grad<N, 2, NFF, C>(su[e][0], sf[e][0][0], sD, m.ialpha, ab);
     }
   }
   __syncthreads();
 
-  // interpolate the 25 face fields to the face Gauss points: along a (the
-  // own trace is read from the volume state; both traces minus the
-  // background in the FP64 layout, whose state is not stored as a
-  // fluctuation -- exact, the values are close), then along b in place;
-  // face coordinates (a, b) = (y, z), (x, z), (x, y) for d = 0, 1, 2.
-  // A line is (element, direction, field, position): w = fl N + x with
-  // fl = (e 3 + d) NFF + f, whose face slot starts at fl NF in sf; the two
-  // sweeps share the decomposition, done once per thread.
+  // interpolate the 21 face fields to the face Gauss points: along a, then
+  // along b, in place; face coordinates (a, b) = (y, z), (x, z), (x, y) for
+  // d = 0, 1, 2.  A line is (element, direction, field, position): w = fl N
+  // + x with fl = (e 3 + d) NFF + f, whose face slot starts at fl NF in sf;
+  // the two sweeps share the decomposition, done once per thread.
   {
     constexpr unsigned SW = 3 * NFF * N;              // lines per element
     constexpr int SL = (E * SW + NT - 1) / NT;        // lines per thread
     C *const sff = &sf[0][0][0][0];
-    unsigned lk[SL];                                  // fl << 3 | x, or ~0 past the block
+    unsigned pa[SL], pb[SL];                          // line starts of the two sweeps, ~0 past the block
 #pragma unroll
     for (int it = 0; it < SL; ++it) {
-      const unsigned w = tid + it * NT, fl = w / N;
-      lk[it] = w < (unsigned)ne * SW ? fl << 3 | (w - fl * N) : ~0u;
+      const unsigned w = tid + it * NT, fl = w / N, x = w - fl * N;
+      const bool in = w < (unsigned)ne * SW;
+      pa[it] = in ? fl * NF + N * x : ~0u;
+      pb[it] = fl * NF + x;
     }
     SymReg<N, C> I;
     I.load(sI);
-    constexpr bool FLUCT = sizeof(TQ) == 4;   // su holds fluctuations (mixed) or the state (FP64)
-    const C brho = FLUCT ? C(0) : C(d.ph.rho0), bE = FLUCT ? C(0) : C(d.ph.E0);
 #pragma unroll
     for (int it = 0; it < SL; ++it) {
-      if (lk[it] == ~0u) continue;
-      const unsigned fl = lk[it] >> 3, b = lk[it] & 7, f = fl % NFF;
+      if (pa[it] == ~0u) continue;
       C v[N], o[N];
-      if (f < NSTATE) {
-        // the face's nodes a + N b of direction dd: (0, a, b), (a, 0, b), (a, b, 0)
-        const unsigned ed = fl / NFF, e = ed / 3, dd = ed - 3 * e;
-        const unsigned as = dd == 0 ? N : 1, bs = dd == 2 ? N : NF;
-        const C *src = &su[e][f][bs * b];
-        const C bg = f == 0 ? brho : f == 4 ? bE : C(0);
 #pragma unroll
-        for (int a = 0; a < N; ++a) v[a] = src[as * a] - bg;
-      } else {
-        // the neighbour's trace is a state too (minus the background in the
-        // FP64 layout), the gradient is not
-        const C bg = f == NSTATE ? brho : f == 2 * NSTATE - 1 ? bE : C(0);
-#pragma unroll
-        for (int a = 0; a < N; ++a) v[a] = sff[fl * NF + a + N * b] - bg;
-      }
+      for (int a = 0; a < N; ++a) v[a] = sff[pa[it] + a];
       sym_apply<N, 1, C>(I.m, v, o);
 #pragma unroll
-      for (int q = 0; q < N; ++q) sff[fl * NF + q + N * b] = o[q];
+      for (int q = 0; q < N; ++q) sff[pa[it] + q] = o[q];
     }
     __syncthreads();
 #pragma unroll
     for (int it = 0; it < SL; ++it) {
-      if (lk[it] == ~0u) continue;
-      C *p = sff + (lk[it] >> 3) * NF + (lk[it] & 7);
+      if (pa[it] == ~0u) continue;
       C v[N], o[N];
 #pragma unroll
-      for (int b = 0; b < N; ++b) v[b] = p[N * b];
+      for (int b = 0; b < N; ++b) v[b] = sff[pb[it] + N * b];
       sym_apply<N, 1, C>(I.m, v, o);
 #pragma unroll
-      for (int q = 0; q < N; ++q) p[N * q] = o[q];
+      for (int q = 0; q < N; ++q) sff[pb[it] + N * q] = o[q];
     }
   }
   __syncthreads();
@@ -620,14 +724,10 @@ ldg_grad_kernel(const UT *__restrict__ U, TQ *__restrict__ Q, TQ *__restrict__ F
       if (r >= (unsigned)ne * NF) continue;
       const unsigned e = r / NF, ab = r - e * NF;
       const C *fp = &sf[e][dd][0][ab];
-      C wN[5], wK[5], q[NGRAD], outK[5], outN[5];
-#pragma unroll
-      for (int s = 0; s < NSTATE; ++s) { wN[s] = fp[s * NF]; wK[s] = fp[(NSTATE + s) * NF]; }
-#pragma unroll
-      for (int k = 0; k < NGRAD; ++k) q[k] = fp[(2 * NSTATE + k) * NF];
-      if (dd == 0) face_flux_pair<C, 0>(wK, wN, q, bg, outK, outN);
-      else if (dd == 1) face_flux_pair<C, 1>(wK, wN, q, bg, outK, outN);
-      else face_flux_pair<C, 2>(wK, wN, q, bg, outK, outN);
+      C outK[5], outN[5];
+      if (dd == 0) face_point_flux<C, 0, NF>(fp, bg, outK, outN);
+      else if (dd == 1) face_point_flux<C, 1, NF>(fp, bg, outK, outN);
+      else face_point_flux<C, 2, NF>(fp, bg, outK, outN);
       // (a select, not an index: a run-time index into m would put it in scratch)
       const double ia = dd == 0 ? m.ialpha[0] : dd == 1 ? m.ialpha[1] : m.ialpha[2];
       const C wt = sw[ab % N] * sw[ab / N] * C(ia);
@@ -644,10 +744,11 @@ ldg_grad_kernel(const UT *__restrict__ U, TQ *__restrict__ Q, TQ *__restrict__ F
 // ---------------------------------------------------------------------
 // Kernel B: face fluxes + volume integrals + inverse mass + RK epilogue.
 //
-// LDS per element: a 20-field volume buffer of type T (which first serves
-// as the face buffer of one direction: 2 sides x 25 fields x N^2 points,
-// and last holds the FP64 inverse-mass sweeps: 5 FP64 fields fit in the
-// space of 10 FP32 fields) and the face residual (5 x N^3, type T).
+// LDS per element: a 20-field volume buffer of type T with room for two
+// face field sets after it (which first holds the face fluxes of the
+// element's 6 faces, and last the FP64 inverse-mass sweeps: 5 FP64 fields
+// fit in the space of 10 FP32 fields) and the face residual (5 x N^3, type
+// T).
 // ---------------------------------------------------------------------
 template <int N, typename T>
 struct RhsLds {
@@ -655,7 +756,8 @@ struct RhsLds {
   static constexpr int NV = NSTATE + NGRAD;          // 20 volume fields
   static constexpr int SV = NV * NN;
   static constexpr int FE = 3 * 2 * NSTATE * NF;     // face fluxes of one element (3 directions, 2 sides)
-  static constexpr int BUF = SV;                     // per element; holds FE first
+  static constexpr int SJ = NSTATE * NF;             // one face field set: the y or z jumps
+  static constexpr int BUF = SV + 2 * SJ;            // per element; holds FE first
   static_assert(FE <= SV, "face fluxes fit the volume buffer");
   // occupancy target (waves per SIMD) handed to the register allocator
   static constexpr int WAVES = sizeof(T) == 4 ? (N <= 5 ? 4 : 2) : (N <= 2 ? 4 : N <= 5 ? 3 : 2);
@@ -668,20 +770,43 @@ struct RhsLds {
 template <int N, typename T>
 struct SOps {
   SymMat<N, T> I, It, Dt;   // interpolation, its transpose, D^T (compute type)
+  SymMat<N, T> Ds[3];       // D = I Dn times 1/alpha_l, l = x, y, z
+  T il0[3][N];              // I L0 times 1/alpha_l
   SymMat<N, double> Mi;     // inverse mass (FP64)
   T w[N];                   // Gauss weights (compute type and FP64)
   double wd[N];
 };
 
+// d = s f in type T (each coefficient rounded once)
 template <int N, typename T>
-__device__ inline void load_sops(SOps<N, T> *so, const Ops<N, double> *op, int tid) {
+__device__ inline void scaled_symmat(SymMat<N, T> &d, const SymMat<N, double> &s, double f) {
+  using M = SymMat<N, T>;
+  for (int q = 0; q < M::QP; ++q) {
+    for (int j = 0; j < M::HP; ++j) { d.P[q][j] = T(s.P[q][j] * f); d.R[q][j] = T(s.R[q][j] * f); }
+    d.C[q] = T(s.C[q] * f);
+  }
+  for (int q = 0; q < M::NA; ++q)
+    for (int j = 0; j < M::NA; ++j) d.A[q][j] = T(s.A[q][j] * f);
+}
+
+template <int N, typename T>
+__device__ inline void load_sops(SOps<N, T> *so, const Ops<N, double> *op, const MeshArgs &m, int tid) {
   if (tid == 0) convert_symmat(so->I, op->sI);
   if (tid == 1) convert_symmat(so->It, op->sIt);
   if (tid == 2) convert_symmat(so->Dt, op->sDt);
   if (tid == 3) so->Mi = op->sMi;
+  if (tid == 4) scaled_symmat(so->Ds[0], op->sD, m.ialpha[0]);
+  if (tid == 5) scaled_symmat(so->Ds[1], op->sD, m.ialpha[1]);
+  if (tid == 6) scaled_symmat(so->Ds[2], op->sD, m.ialpha[2]);
   if (tid >= 64 && tid < 64 + N) {
     so->w[tid - 64] = T(op->w[tid - 64]);
     so->wd[tid - 64] = op->w[tid - 64];
+  }
+  if (tid >= 128 && tid < 128 + N) {
+    const int q = tid - 128;
+    so->il0[0][q] = T(op->IL0[q] * m.ialpha[0]);
+    so->il0[1][q] = T(op->IL0[q] * m.ialpha[1]);
+    so->il0[2][q] = T(op->IL0[q] * m.ialpha[2]);
   }
 }
 
@@ -770,35 +895,10 @@ __device__ inline void rhs_faces(const T *__restrict__ FF, const long (*snb)[6],
   }
 }
 
-// Work mapping of the volume-line loops (E elements x X = 20 fields x N^2
-// lines): per-element chunks of NT lines when one element fills at least
-// 3/4 of its chunks (p >= 4) -- the element of an item is then a
-// compile-time constant and the line decomposition, which depends only on
-// the thread and the chunk, is shared by all elements -- otherwise the
-// lines of all elements packed densely.  Items past the block have e >= ne
-// or r >= X.
-template <int N, int E>
-struct VolMap {
-  static constexpr int X = (NSTATE + NGRAD) * N * N;
-  static constexpr int C = (X + NT - 1) / NT;
-  static constexpr bool PER_E = 4 * X >= 3 * C * NT;
-  static constexpr int ITEMS = PER_E ? E * C : (E * X + NT - 1) / NT;
-  __device__ static inline void item(int it, int tid, int &e, int &r) {
-    if (PER_E) {
-      e = it / C;
-      r = tid + (it % C) * NT;
-    } else {
-      const int w = tid + it * NT;
-      e = w / X;
-      r = w - e * X;
-    }
-  }
-};
-
 template <int N, int E, typename T, typename UT>
 __global__ void __launch_bounds__(NT, (RhsLds<N, T>::WAVES))
-ldg_rhs_kernel(const UT *__restrict__ U, const T *__restrict__ Q, const T *__restrict__ FF, LaunchArgs la,
-               const Ops<N, double> *__restrict__ opp, StageArgs st) {
+ldg_rhs_kernel(const UT *__restrict__ U, const T *__restrict__ FF, LaunchArgs la, const Ops<N, double> *__restrict__ opp,
+               StageArgs st) {
   using L = RhsLds<N, T>;
   constexpr int NN = L::NN, NF = L::NF, BUF = L::BUF;
   __shared__ __align__(16) T sbuf[E][BUF];
@@ -835,7 +935,7 @@ ldg_rhs_kernel(const UT *__restrict__ U, const T *__restrict__ Q, const T *__res
   auto md = [&](int e) -> double * { return reinterpret_cast<double *>(&sbuf[e][0]); };
 
   fill_neighbors(m, ec, d.elem_off, ne, snb, tid);
-  load_sops<N, T>(&sops, opp, tid);
+  load_sops<N, T>(&sops, opp, m, tid);
   for (int w = tid; w < ne * NSTATE * NN; w += NT) (&sr[0][0][0])[w] = T(0);
   __syncthreads();
 
@@ -843,89 +943,185 @@ ldg_rhs_kernel(const UT *__restrict__ U, const T *__restrict__ Q, const T *__res
   rhs_faces<N, E, T>(FF, snb, k0, ne, so, sbuf, sr, tid);
 
   // ================= volume =================
-  // load U (minus the background) and Q x-lines from HBM, all loads of the
-  // thread first, and interpolate along x
-  using VM = VolMap<N, E>;
-  // start of a volume line in its element's field block (field f, line ab):
-  // f NN + line_base(dd, a, b).  With the per-element mapping the thread's
-  // lines are the same in every element: decomposed once, for x, y and z.
-  int vl[3][VM::PER_E ? VM::C : 1];
-  if constexpr (VM::PER_E) {
-#pragma unroll
-    for (int c = 0; c < VM::C; ++c) {
-      const int r = tid + c * NT, f = r / NF, ab = r % NF;
-#pragma unroll
-      for (int dd = 0; dd < 3; ++dd) vl[dd][c] = f * NN + line_base<N>(dd, ab % N, ab / N);
-    }
-  }
-  auto line_start = [&](int it, int dd, int r) -> int {
-    if constexpr (VM::PER_E) {
-      return vl[dd][it % VM::C];
-    } else {
-      const int ab = r % NF;
-      return (r / NF) * NN + line_base<N>(dd, ab % N, ab / N);
-    }
-  };
+  // The state and its gradient at the volume Gauss points, from the state
+  // alone.  The gradient kernel's Q_l = (1/alpha_l) (Dn u + L0 J_l) along
+  // direction l (J_l: the jump u - u_nbr on the element's -l face, the only
+  // face whose neighbour trace enters the one-sided LDG gradient),
+  // interpolated to the Gauss points, factorises as
+  //   Q_l = (1/alpha_l) [ D_l (x) I (x) I u + (I L0)_l (x) (I (x) I) J_l ]
+  // with D = I Dn, the basis derivatives at the Gauss points (exact: the
+  // derivative of a basis polynomial lies in their span).  Sweeps:
+  //   x (lines from HBM):  X = I u,  Q_x <- D u + IL0 J_x;  J_y, J_z <- I J
+  //   y:  XY = I X,  Q_y <- D X + IL0 (I J_y);  Q_x <- I Q_x;  J_z <- I J_z
+  //   z:  u_g = I XY,  Q_z <- D XY + IL0 (I I J_z);  Q_x, Q_y <- I Q
+  // (45 field sweeps instead of 60 for interpolating a stored gradient, and
+  // no gradient traffic).  Fields of the volume buffer: u 0-4, Q_l 5+5l..;
+  // the y and z jumps after them.  In type T on fluctuations about the
+  // background, as the face terms.
+  constexpr int XL = NSTATE * NF;                    // lines of one direction per element
+  auto sjy = [&](int e) -> T * { return &sbuf[e][L::SV]; };
+  auto sjz = [&](int e) -> T * { return &sbuf[e][L::SV + L::SJ]; };
   {
-    T v[VM::ITEMS][N];
+    constexpr int XI = (E * XL + NT - 1) / NT;
+    const unsigned totx = ne * XL;
+    UT vo[XI][N], vx[XI], vy[XI][N], vz[XI][N];
+    // all of the thread's loads first: its own x-lines, the -x neighbour's
+    // trace at each, and on the lines of the -y / -z faces the neighbour's
+    // x-line along that face
 #pragma unroll
-    for (int it = 0; it < VM::ITEMS; ++it) {
-      int e, r;
-      VM::item(it, tid, e, r);
-      if (e < ne && r < VM::X) {
-        const int o = line_start(it, 0, r);   // f NN + N (y, z line index): also the line's offset in HBM
-        if (o < NSTATE * NN) {
-          const UT *src = U + (k0 + e) * (NSTATE * NN) + o;
-          const double bgs = o < NN ? brho : o >= 4 * NN ? bE : 0.0;
+    for (int it = 0; it < XI; ++it) {
+      const unsigned w = tid + it * NT < totx ? tid + it * NT : totx - 1;
+      const unsigned e = w / XL, r = w - e * XL, s = r / NF, ab = r - s * NF;
+      const unsigned iy = ab % N, iz = ab / N;
+      const UT *src = U + (k0 + e) * (NSTATE * NN) + (s * NN + N * ab);
 #pragma unroll
-          for (int i = 0; i < N; ++i) v[it][i] = T((double)src[i] - bgs);
-        } else {
-          const T *src = Q + (k0 + e) * (NGRAD * NN) + (o - NSTATE * NN);
+      for (int i = 0; i < N; ++i) vo[it][i] = src[i];
+      vx[it] = U[snb[e][0] * (NSTATE * NN) + (s * NN + N * ab + (N - 1))];
+      if (iy == 0) {
+        const UT *ny = U + snb[e][2] * (NSTATE * NN) + (s * NN + N * (N - 1) + NF * iz);
 #pragma unroll
-          for (int i = 0; i < N; ++i) v[it][i] = src[i];
-        }
+        for (int i = 0; i < N; ++i) vy[it][i] = ny[i];
+      }
+      if (iz == 0) {
+        const UT *nz = U + snb[e][4] * (NSTATE * NN) + (s * NN + NF * (N - 1) + N * iy);
+#pragma unroll
+        for (int i = 0; i < N; ++i) vz[it][i] = nz[i];
       }
     }
-    SymReg<N, T> I;
+    SymReg<N, T> I, Dx;
     I.load(so->I);
+    Dx.load(so->Ds[0]);
+    T l0[N];
 #pragma unroll
-    for (int it = 0; it < VM::ITEMS; ++it) {
-      int e, r;
-      VM::item(it, tid, e, r);
-      if (e < ne && r < VM::X) {
-        T *dst = &sbuf[e][line_start(it, 0, r)];
-        T o[N];
-        sym_apply<N, 1, T>(I.m, v[it], o);
+    for (int q = 0; q < N; ++q) l0[q] = so->il0[0][q];
 #pragma unroll
-        for (int q = 0; q < N; ++q) dst[q] = o[q];
+    for (int it = 0; it < XI; ++it) {
+      if (tid + it * NT >= totx) continue;
+      const unsigned w = tid + it * NT;
+      const unsigned e = w / XL, r = w - e * XL, s = r / NF, ab = r - s * NF;
+      const unsigned iy = ab % N, iz = ab / N;
+      const double bgs = s == 0 ? brho : s == 4 ? bE : 0.0;
+      auto fl = [&](UT x) -> T { return FLUCT ? T(x) : T((double)x - bgs); };
+      T u[N], o[N], g[N];
+#pragma unroll
+      for (int i = 0; i < N; ++i) u[i] = fl(vo[it][i]);
+      const T jx = u[0] - fl(vx[it]);
+      sym_apply<N, 1, T>(I.m, u, o);
+      sym_apply<N, -1, T>(Dx.m, u, g);
+      T *px = &sbuf[e][s * NN + N * ab];
+      T *pq = &sbuf[e][(NSTATE + s) * NN + N * ab];
+#pragma unroll
+      for (int q = 0; q < N; ++q) {
+        px[q] = o[q];
+        pq[q] = g[q] + l0[q] * jx;
+      }
+      if (iy == 0) {
+        T j[N];
+#pragma unroll
+        for (int i = 0; i < N; ++i) j[i] = u[i] - fl(vy[it][i]);
+        sym_apply<N, 1, T>(I.m, j, o);
+#pragma unroll
+        for (int q = 0; q < N; ++q) sjy(e)[s * NF + q + N * iz] = o[q];
+      }
+      if (iz == 0) {
+        T j[N];
+#pragma unroll
+        for (int i = 0; i < N; ++i) j[i] = u[i] - fl(vz[it][i]);
+        sym_apply<N, 1, T>(I.m, j, o);
+#pragma unroll
+        for (int q = 0; q < N; ++q) sjz(e)[s * NF + q + N * iy] = o[q];
       }
     }
   }
   __syncthreads();
-  // y then z sweeps (in place)
+  // y sweeps
+  {
+    SymReg<N, T> I, Dy;
+    I.load(so->I);
+    Dy.load(so->Ds[1]);
+    T l0[N];
 #pragma unroll
-  for (int dd = 1; dd < 3; ++dd) {
-    {
-      SymReg<N, T> I;
-      I.load(so->I);
-      const int ls = line_stride<N>(dd);
+    for (int q = 0; q < N; ++q) l0[q] = so->il0[1][q];
+    // I X (in place) and D X + IL0 J_y into the Q_y slot; line (s, x, z)
+    for (unsigned w = tid; w < (unsigned)ne * XL; w += NT) {
+      const unsigned e = w / XL, r = w - e * XL, s = r / NF, ab = r - s * NF;
+      const unsigned x = ab % N, z = ab / N;
+      T *p = &sbuf[e][s * NN + x + NF * z];
+      T *pq = &sbuf[e][(NSTATE + NSTATE + s) * NN + x + NF * z];
+      const T jy = sjy(e)[s * NF + ab];
+      T v[N], o[N], g[N];
 #pragma unroll
-      for (int it = 0; it < VM::ITEMS; ++it) {
-        int e, r;
-        VM::item(it, tid, e, r);
-        if (e < ne && r < VM::X) {
-          T *p = &sbuf[e][line_start(it, dd, r)];
-          T v[N], o[N];
+      for (int i = 0; i < N; ++i) v[i] = p[N * i];
+      sym_apply<N, 1, T>(I.m, v, o);
+      sym_apply<N, -1, T>(Dy.m, v, g);
 #pragma unroll
-          for (int i = 0; i < N; ++i) v[i] = p[i * ls];
-          sym_apply<N, 1, T>(I.m, v, o);
-#pragma unroll
-          for (int q = 0; q < N; ++q) p[q * ls] = o[q];
-        }
+      for (int q = 0; q < N; ++q) {
+        p[N * q] = o[q];
+        pq[N * q] = g[q] + l0[q] * jy;
       }
     }
-    __syncthreads();
+    // I Q_x (in place)
+    for (unsigned w = tid; w < (unsigned)ne * XL; w += NT) {
+      const unsigned e = w / XL, r = w - e * XL, s = r / NF, ab = r - s * NF;
+      T *p = &sbuf[e][(NSTATE + s) * NN + ab % N + NF * (ab / N)];
+      T v[N], o[N];
+#pragma unroll
+      for (int i = 0; i < N; ++i) v[i] = p[N * i];
+      sym_apply<N, 1, T>(I.m, v, o);
+#pragma unroll
+      for (int q = 0; q < N; ++q) p[N * q] = o[q];
+    }
+    // I J_z along y (in place): line (s, x)
+    for (unsigned w = tid; w < (unsigned)ne * NSTATE * N; w += NT) {
+      const unsigned e = w / (NSTATE * N), r = w - e * (NSTATE * N);
+      T *p = sjz(e) + (r / N) * NF + r % N;
+      T v[N], o[N];
+#pragma unroll
+      for (int i = 0; i < N; ++i) v[i] = p[N * i];
+      sym_apply<N, 1, T>(I.m, v, o);
+#pragma unroll
+      for (int q = 0; q < N; ++q) p[N * q] = o[q];
+    }
   }
+  __syncthreads();
+  // z sweeps
+  {
+    SymReg<N, T> I, Dz;
+    I.load(so->I);
+    Dz.load(so->Ds[2]);
+    T l0[N];
+#pragma unroll
+    for (int q = 0; q < N; ++q) l0[q] = so->il0[2][q];
+    // I XY (in place) and D XY + IL0 J_z into the Q_z slot; line (s, x, y)
+    for (unsigned w = tid; w < (unsigned)ne * XL; w += NT) {
+      const unsigned e = w / XL, r = w - e * XL, s = r / NF, ab = r - s * NF;
+      T *p = &sbuf[e][s * NN + ab];
+      T *pq = &sbuf[e][(NSTATE + 2 * NSTATE + s) * NN + ab];
+      const T jz = sjz(e)[s * NF + ab];
+      T v[N], o[N], g[N];
+#pragma unroll
+      for (int i = 0; i < N; ++i) v[i] = p[NF * i];
+      sym_apply<N, 1, T>(I.m, v, o);
+      sym_apply<N, -1, T>(Dz.m, v, g);
+#pragma unroll
+      for (int q = 0; q < N; ++q) {
+        p[NF * q] = o[q];
+        pq[NF * q] = g[q] + l0[q] * jz;
+      }
+    }
+    // I Q_x and I Q_y (in place)
+    for (unsigned w = tid; w < (unsigned)ne * 2 * XL; w += NT) {
+      const unsigned e = w / (2 * XL), r = w - e * (2 * XL), f = r / NF, ab = r - f * NF;
+      T *p = &sbuf[e][(NSTATE + f) * NN + ab];
+      T v[N], o[N];
+#pragma unroll
+      for (int i = 0; i < N; ++i) v[i] = p[NF * i];
+      sym_apply<N, 1, T>(I.m, v, o);
+#pragma unroll
+      for (int q = 0; q < N; ++q) p[NF * q] = o[q];
+    }
+  }
+  __syncthreads();
 
   // pointwise volume flux, weighted; optional Gauss diagnostics (FP64 sums)
   double dke = 0.0, dens = 0.0;

@@ -105,7 +105,7 @@ def test_noalloc_rejects_bad_arguments_before_any_device_work():
     cfg = ldg_amd.ldg_config()
     lib.ldg_config_default(ctypes.byref(cfg))
     assert lib.ldg_rhs_noalloc(None, None, None, None) == ldg_amd.LDG_EINVAL
-    cfg.p, cfg.nx, cfg.ny, cfg.nz = 4, 15, 15, 15          # beyond LDG_NOALLOC_SCRATCH_BYTES in FP64
+    cfg.p, cfg.nx, cfg.ny, cfg.nz = 4, 23, 23, 23          # beyond LDG_NOALLOC_SCRATCH_BYTES in FP64
     dummy = ctypes.c_void_p(8)                              # never dereferenced: refused first
     assert lib.ldg_rhs_noalloc(ctypes.byref(cfg), dummy, dummy, None) == ldg_amd.LDG_EINVAL
     assert b"LDG_NOALLOC_SCRATCH_BYTES" in lib.ldg_last_error(None)

@@ -75,8 +75,8 @@ def test_noalloc_refuses_meshes_beyond_its_scratch():
     hip = _hip()
     g = ldg_amd.LDG(4, 3)
     cfg = g.cfg
-    cfg.nx = cfg.ny = cfg.nz = 15            # 15^3 x (15 x 125 + 30 x 25) x 8 bytes > 64 MiB
-    nbytes = 15 ** 3 * 5 * 125 * 8            # full-size device arrays, although the call is refused first
+    cfg.nx = cfg.ny = cfg.nz = 23            # 8192 + 23^3 x 30 x 25 x 8 bytes > 64 MiB
+    nbytes = 23 ** 3 * 5 * 125 * 8            # full-size device arrays, although the call is refused first
     du, dk = ctypes.c_void_p(), ctypes.c_void_p()
     assert hip.hipMalloc(ctypes.byref(du), nbytes) == 0 and hip.hipMalloc(ctypes.byref(dk), nbytes) == 0
     try:
@@ -87,9 +87,9 @@ def test_noalloc_refuses_meshes_beyond_its_scratch():
         hip.hipFree(dk)
     assert e.value.code == ldg_amd.LDG_EINVAL
     assert "LDG_NOALLOC_SCRATCH_BYTES" in str(e.value)
-    # the largest cubic FP64 mesh at p = 4 that fits: 14^3
-    cfg.nx = cfg.ny = cfg.nz = 14
-    u = _state(4, 14, 3)
+    # the largest cubic FP64 mesh at p = 4 that fits: 22^3
+    cfg.nx = cfg.ny = cfg.nz = 22
+    u = _state(4, 22, 3)
     k = _noalloc_rhs(hip, cfg, u)
-    g14 = ldg_amd.LDG(4, 14)
-    assert np.array_equal(k, g14.rhs(u))
+    g22 = ldg_amd.LDG(4, 22)
+    assert np.array_equal(k, g22.rhs(u))
