@@ -934,10 +934,57 @@ ldg_rhs_kernel(const UT *__restrict__ U, const T *__restrict__ FF, LaunchArgs la
   // sweeps and the epilogue
   auto md = [&](int e) -> double * { return reinterpret_cast<double *>(&sbuf[e][0]); };
 
+  // The volume lines of the x sweep stage (below): item i = (e, s, ab) of
+  // the block's ne x 5 x N^2 x-lines, in wave-aligned ranges of KP items per
+  // kind.  Their HBM loads are issued here, so that they land while the
+  // face terms are lifted.  Kind 0 computes I u and the y and z jumps, kind
+  // 1 D u and the x jump; when both kinds fit the workgroup side by side
+  // (one element per workgroup at p = 4) they run on separate waves,
+  // otherwise every item does both.
+  constexpr int XL = NSTATE * NF;                    // lines of one direction per element
+  constexpr unsigned KP = (E * XL + 63) / 64 * 64;   // items of one kind, padded to whole waves
+  constexpr bool XSPLIT = 2 * KP <= NT;
+  constexpr int XI = XSPLIT ? 1 : (int)((KP + NT - 1) / NT);
+  const unsigned totx = ne * XL;
+  UT vo[XI][N], vx[XI], vy[XI][N], vz[XI][N];
+  unsigned xi[XI];                                   // the item, or ~0
+  auto x_item = [&](int it) -> unsigned {
+    const unsigned w = tid + it * NT, i = XSPLIT ? w % KP : w;
+    return i < totx ? i : ~0u;
+  };
+#pragma unroll
+  for (int it = 0; it < XI; ++it) {
+    xi[it] = x_item(it);
+    const unsigned i = xi[it] == ~0u ? totx - 1 : xi[it];
+    const unsigned e = i / XL, r = i - e * XL;       // r = s NF + ab: the line starts at N r in its element
+    const UT *src = U + (k0 + e) * (NSTATE * NN) + N * r;
+#pragma unroll
+    for (int q = 0; q < N; ++q) vo[it][q] = src[q];
+  }
+
   fill_neighbors(m, ec, d.elem_off, ne, snb, tid);
   load_sops<N, T>(&sops, opp, m, tid);
   for (int w = tid; w < ne * NSTATE * NN; w += NT) (&sr[0][0][0])[w] = T(0);
   __syncthreads();
+  // the neighbours' traces the jumps need: the -x neighbour's node at the
+  // line's start, and the -y (-z) neighbour's x-line along the -y (-z) face;
+  // the latter loaded by every item (from the line itself where the line is
+  // not on that face: unconditional loads are not waited for at a branch)
+#pragma unroll
+  for (int it = 0; it < XI; ++it) {
+    const unsigned i = xi[it] == ~0u ? totx - 1 : xi[it];
+    const unsigned e = i / XL, r = i - e * XL, s = r / NF, ab = r - s * NF;
+    const unsigned iy = ab % N, iz = ab / N;
+    const UT *own = U + (k0 + e) * (NSTATE * NN) + N * r;
+    vx[it] = U[snb[e][0] * (NSTATE * NN) + (N * r + (N - 1))];
+    const UT *ny = iy == 0 ? U + snb[e][2] * (NSTATE * NN) + (s * NN + N * (N - 1) + NF * iz) : own;
+    const UT *nz = iz == 0 ? U + snb[e][4] * (NSTATE * NN) + (s * NN + NF * (N - 1) + N * iy) : own;
+#pragma unroll
+    for (int q = 0; q < N; ++q) {
+      vy[it][q] = ny[q];
+      vz[it][q] = nz[q];
+    }
+  }
 
   // ================= faces: lift the gradient kernel's fluxes =================
   rhs_faces<N, E, T>(FF, snb, k0, ne, so, sbuf, sr, tid);
@@ -957,37 +1004,13 @@ ldg_rhs_kernel(const UT *__restrict__ U, const T *__restrict__ FF, LaunchArgs la
   // (45 field sweeps instead of 60 for interpolating a stored gradient, and
   // no gradient traffic).  Fields of the volume buffer: u 0-4, Q_l 5+5l..;
   // the y and z jumps after them.  In type T on fluctuations about the
-  // background, as the face terms.
-  constexpr int XL = NSTATE * NF;                    // lines of one direction per element
+  // background, as the face terms.  A line item (e, s, ab) names the x-line
+  // (y, z) = (a, b), the y-line (x, z) = (a, b) and the z-line (x, y) =
+  // (a, b) of field s; each stage runs its kinds of lines in wave-aligned
+  // ranges of KP items.
   auto sjy = [&](int e) -> T * { return &sbuf[e][L::SV]; };
   auto sjz = [&](int e) -> T * { return &sbuf[e][L::SV + L::SJ]; };
   {
-    constexpr int XI = (E * XL + NT - 1) / NT;
-    const unsigned totx = ne * XL;
-    UT vo[XI][N], vx[XI], vy[XI][N], vz[XI][N];
-    // all of the thread's loads first: its own x-lines, the -x neighbour's
-    // trace at each, and on the lines of the -y / -z faces the neighbour's
-    // x-line along that face
-#pragma unroll
-    for (int it = 0; it < XI; ++it) {
-      const unsigned w = tid + it * NT < totx ? tid + it * NT : totx - 1;
-      const unsigned e = w / XL, r = w - e * XL, s = r / NF, ab = r - s * NF;
-      const unsigned iy = ab % N, iz = ab / N;
-      const UT *src = U + (k0 + e) * (NSTATE * NN) + (s * NN + N * ab);
-#pragma unroll
-      for (int i = 0; i < N; ++i) vo[it][i] = src[i];
-      vx[it] = U[snb[e][0] * (NSTATE * NN) + (s * NN + N * ab + (N - 1))];
-      if (iy == 0) {
-        const UT *ny = U + snb[e][2] * (NSTATE * NN) + (s * NN + N * (N - 1) + NF * iz);
-#pragma unroll
-        for (int i = 0; i < N; ++i) vy[it][i] = ny[i];
-      }
-      if (iz == 0) {
-        const UT *nz = U + snb[e][4] * (NSTATE * NN) + (s * NN + NF * (N - 1) + N * iy);
-#pragma unroll
-        for (int i = 0; i < N; ++i) vz[it][i] = nz[i];
-      }
-    }
     SymReg<N, T> I, Dx;
     I.load(so->I);
     Dx.load(so->Ds[0]);
@@ -996,45 +1019,49 @@ ldg_rhs_kernel(const UT *__restrict__ U, const T *__restrict__ FF, LaunchArgs la
     for (int q = 0; q < N; ++q) l0[q] = so->il0[0][q];
 #pragma unroll
     for (int it = 0; it < XI; ++it) {
-      if (tid + it * NT >= totx) continue;
-      const unsigned w = tid + it * NT;
-      const unsigned e = w / XL, r = w - e * XL, s = r / NF, ab = r - s * NF;
+      if (xi[it] == ~0u) continue;
+      const unsigned i = xi[it];
+      const unsigned e = i / XL, r = i - e * XL, s = r / NF, ab = r - s * NF;
       const unsigned iy = ab % N, iz = ab / N;
       const double bgs = s == 0 ? brho : s == 4 ? bE : 0.0;
       auto fl = [&](UT x) -> T { return FLUCT ? T(x) : T((double)x - bgs); };
-      T u[N], o[N], g[N];
+      const bool k0s = !XSPLIT || tid < KP, k1s = !XSPLIT || tid >= KP;   // kinds of this item
+      T u[N], o[N];
 #pragma unroll
-      for (int i = 0; i < N; ++i) u[i] = fl(vo[it][i]);
-      const T jx = u[0] - fl(vx[it]);
-      sym_apply<N, 1, T>(I.m, u, o);
-      sym_apply<N, -1, T>(Dx.m, u, g);
-      T *px = &sbuf[e][s * NN + N * ab];
-      T *pq = &sbuf[e][(NSTATE + s) * NN + N * ab];
+      for (int q = 0; q < N; ++q) u[q] = fl(vo[it][q]);
+      T *px = &sbuf[e][N * r];
+      if (k0s) {
+        sym_apply<N, 1, T>(I.m, u, o);
 #pragma unroll
-      for (int q = 0; q < N; ++q) {
-        px[q] = o[q];
-        pq[q] = g[q] + l0[q] * jx;
+        for (int q = 0; q < N; ++q) px[q] = o[q];
+        if (iy == 0) {
+          T j[N];
+#pragma unroll
+          for (int q = 0; q < N; ++q) j[q] = u[q] - fl(vy[it][q]);
+          sym_apply<N, 1, T>(I.m, j, o);
+#pragma unroll
+          for (int q = 0; q < N; ++q) sjy(e)[s * NF + q + N * iz] = o[q];
+        }
+        if (iz == 0) {
+          T j[N];
+#pragma unroll
+          for (int q = 0; q < N; ++q) j[q] = u[q] - fl(vz[it][q]);
+          sym_apply<N, 1, T>(I.m, j, o);
+#pragma unroll
+          for (int q = 0; q < N; ++q) sjz(e)[s * NF + q + N * iy] = o[q];
+        }
       }
-      if (iy == 0) {
-        T j[N];
+      if (k1s) {
+        const T jx = u[0] - fl(vx[it]);
+        sym_apply<N, -1, T>(Dx.m, u, o);
 #pragma unroll
-        for (int i = 0; i < N; ++i) j[i] = u[i] - fl(vy[it][i]);
-        sym_apply<N, 1, T>(I.m, j, o);
-#pragma unroll
-        for (int q = 0; q < N; ++q) sjy(e)[s * NF + q + N * iz] = o[q];
-      }
-      if (iz == 0) {
-        T j[N];
-#pragma unroll
-        for (int i = 0; i < N; ++i) j[i] = u[i] - fl(vz[it][i]);
-        sym_apply<N, 1, T>(I.m, j, o);
-#pragma unroll
-        for (int q = 0; q < N; ++q) sjz(e)[s * NF + q + N * iy] = o[q];
+        for (int q = 0; q < N; ++q) px[NSTATE * NN + q] = o[q] + l0[q] * jx;
       }
     }
   }
   __syncthreads();
-  // y sweeps
+  // y sweeps: kind 0 I X (in place) and D X + IL0 J_y into the Q_y slot;
+  // kind 1 I Q_x (in place); then I J_z along y
   {
     SymReg<N, T> I, Dy;
     I.load(so->I);
@@ -1042,49 +1069,40 @@ ldg_rhs_kernel(const UT *__restrict__ U, const T *__restrict__ FF, LaunchArgs la
     T l0[N];
 #pragma unroll
     for (int q = 0; q < N; ++q) l0[q] = so->il0[1][q];
-    // I X (in place) and D X + IL0 J_y into the Q_y slot; line (s, x, z)
-    for (unsigned w = tid; w < (unsigned)ne * XL; w += NT) {
-      const unsigned e = w / XL, r = w - e * XL, s = r / NF, ab = r - s * NF;
-      const unsigned x = ab % N, z = ab / N;
-      T *p = &sbuf[e][s * NN + x + NF * z];
-      T *pq = &sbuf[e][(NSTATE + NSTATE + s) * NN + x + NF * z];
-      const T jy = sjy(e)[s * NF + ab];
-      T v[N], o[N], g[N];
-#pragma unroll
-      for (int i = 0; i < N; ++i) v[i] = p[N * i];
-      sym_apply<N, 1, T>(I.m, v, o);
-      sym_apply<N, -1, T>(Dy.m, v, g);
-#pragma unroll
-      for (int q = 0; q < N; ++q) {
-        p[N * q] = o[q];
-        pq[N * q] = g[q] + l0[q] * jy;
-      }
-    }
-    // I Q_x (in place)
-    for (unsigned w = tid; w < (unsigned)ne * XL; w += NT) {
-      const unsigned e = w / XL, r = w - e * XL, s = r / NF, ab = r - s * NF;
-      T *p = &sbuf[e][(NSTATE + s) * NN + ab % N + NF * (ab / N)];
+    for (unsigned w = tid; w < 2 * KP; w += NT) {
+      const unsigned kind = w / KP, i = w - kind * KP;
+      if (i >= totx) continue;
+      const unsigned e = i / XL, r = i - e * XL, s = r / NF, ab = r - s * NF;
+      const unsigned a = ab % N, b = ab / N;
+      T *p = &sbuf[e][(kind == 0 ? s : NSTATE + s) * NN + a + NF * b];
       T v[N], o[N];
 #pragma unroll
-      for (int i = 0; i < N; ++i) v[i] = p[N * i];
+      for (int q = 0; q < N; ++q) v[q] = p[N * q];
       sym_apply<N, 1, T>(I.m, v, o);
 #pragma unroll
       for (int q = 0; q < N; ++q) p[N * q] = o[q];
+      if (kind == 0) {
+        const T jy = sjy(e)[s * NF + ab];
+        sym_apply<N, -1, T>(Dy.m, v, o);
+        T *pq = p + 2 * NSTATE * NN;
+#pragma unroll
+        for (int q = 0; q < N; ++q) pq[N * q] = o[q] + l0[q] * jy;
+      }
     }
-    // I J_z along y (in place): line (s, x)
     for (unsigned w = tid; w < (unsigned)ne * NSTATE * N; w += NT) {
       const unsigned e = w / (NSTATE * N), r = w - e * (NSTATE * N);
       T *p = sjz(e) + (r / N) * NF + r % N;
       T v[N], o[N];
 #pragma unroll
-      for (int i = 0; i < N; ++i) v[i] = p[N * i];
+      for (int q = 0; q < N; ++q) v[q] = p[N * q];
       sym_apply<N, 1, T>(I.m, v, o);
 #pragma unroll
       for (int q = 0; q < N; ++q) p[N * q] = o[q];
     }
   }
   __syncthreads();
-  // z sweeps
+  // z sweeps: kind 0 I XY (in place) and D XY + IL0 J_z into the Q_z slot;
+  // kinds 1 and 2 I Q_x and I Q_y (in place)
   {
     SymReg<N, T> I, Dz;
     I.load(so->I);
@@ -1092,33 +1110,24 @@ ldg_rhs_kernel(const UT *__restrict__ U, const T *__restrict__ FF, LaunchArgs la
     T l0[N];
 #pragma unroll
     for (int q = 0; q < N; ++q) l0[q] = so->il0[2][q];
-    // I XY (in place) and D XY + IL0 J_z into the Q_z slot; line (s, x, y)
-    for (unsigned w = tid; w < (unsigned)ne * XL; w += NT) {
-      const unsigned e = w / XL, r = w - e * XL, s = r / NF, ab = r - s * NF;
-      T *p = &sbuf[e][s * NN + ab];
-      T *pq = &sbuf[e][(NSTATE + 2 * NSTATE + s) * NN + ab];
-      const T jz = sjz(e)[s * NF + ab];
-      T v[N], o[N], g[N];
-#pragma unroll
-      for (int i = 0; i < N; ++i) v[i] = p[NF * i];
-      sym_apply<N, 1, T>(I.m, v, o);
-      sym_apply<N, -1, T>(Dz.m, v, g);
-#pragma unroll
-      for (int q = 0; q < N; ++q) {
-        p[NF * q] = o[q];
-        pq[NF * q] = g[q] + l0[q] * jz;
-      }
-    }
-    // I Q_x and I Q_y (in place)
-    for (unsigned w = tid; w < (unsigned)ne * 2 * XL; w += NT) {
-      const unsigned e = w / (2 * XL), r = w - e * (2 * XL), f = r / NF, ab = r - f * NF;
-      T *p = &sbuf[e][(NSTATE + f) * NN + ab];
+    for (unsigned w = tid; w < 3 * KP; w += NT) {
+      const unsigned kind = w / KP, i = w - kind * KP;
+      if (i >= totx) continue;
+      const unsigned e = i / XL, r = i - e * XL, s = r / NF, ab = r - s * NF;
+      T *p = &sbuf[e][(kind * NSTATE + s) * NN + ab];
       T v[N], o[N];
 #pragma unroll
-      for (int i = 0; i < N; ++i) v[i] = p[NF * i];
+      for (int q = 0; q < N; ++q) v[q] = p[NF * q];
       sym_apply<N, 1, T>(I.m, v, o);
 #pragma unroll
       for (int q = 0; q < N; ++q) p[NF * q] = o[q];
+      if (kind == 0) {
+        const T jz = sjz(e)[s * NF + ab];
+        sym_apply<N, -1, T>(Dz.m, v, o);
+        T *pq = p + 3 * NSTATE * NN;
+#pragma unroll
+        for (int q = 0; q < N; ++q) pq[NF * q] = o[q] + l0[q] * jz;
+      }
     }
   }
   __syncthreads();
