@@ -421,10 +421,19 @@ struct GradWaves {
   static constexpr int value = want < by_lds ? want : by_lds;
 };
 
-// Face-flux array: per element, 3 directions x 2 sides x 5 states x N^2
-// face points.
-template <int N>
-__host__ __device__ constexpr int ff_per_element() { return 3 * 2 * NSTATE * N * N; }
+// Face-flux array: per element, 3 directions x 2 sides, each a chunk of 5
+// states x N^2 face points padded to a whole number of 16-byte words (the
+// rhs kernel reads it in 16-byte loads).
+__host__ __device__ constexpr int ff_chunk(int n, int bytes) {
+  return (NSTATE * n * n * bytes + 15) / 16 * 16 / bytes;
+}
+template <int N, typename T>
+__host__ __device__ constexpr int ff_per_element() { return 6 * ff_chunk(N, (int)sizeof(T)); }
+
+// 16-byte vector of T
+template <typename T> struct Vec16;
+template <> struct Vec16<float> { using type = float4; };
+template <> struct Vec16<double> { using type = double2; };
 
 // Face fluxes of direction D at one face point: fp points at the point's
 // slot of field 0 (fields NF apart: own trace 0-4, neighbour's trace 5-9,
@@ -523,7 +532,11 @@ ldg_grad_kernel(const UT *__restrict__ U, TQ *__restrict__ Q, TQ *__restrict__ F
     const int i = tid / (N + 1), j = tid % (N + 1);
     sD[i][j] = C(j < N ? opp->Dn[i][j] + (j == 0 ? opp->L0[i] : 0.0) : -opp->L0[i]);
   }
-  if (tid == 64) convert_symmat(sI, opp->sI);
+  {
+    constexpr int MS = sizeof(SymMat<N, C>) / sizeof(C);   // coefficients of I, one per thread
+    if (tid >= 64 && tid < 64 + MS)
+      reinterpret_cast<C *>(&sI)[tid - 64] = C(reinterpret_cast<const double *>(&opp->sI)[tid - 64]);
+  }
   if (tid >= 128 && tid < 128 + N) sw[tid - 128] = C(opp->w[tid - 128]);
   __syncthreads();   // neighbour table
 
@@ -731,11 +744,12 @@ ldg_grad_kernel(const UT *__restrict__ U, TQ *__restrict__ Q, TQ *__restrict__ F
       // (a select, not an index: a run-time index into m would put it in scratch)
       const double ia = dd == 0 ? m.ialpha[0] : dd == 1 ? m.ialpha[1] : m.ialpha[2];
       const C wt = sw[ab % N] * sw[ab / N] * C(ia);
-      TQ *dst = FF + (k0 + e) * ff_per_element<N>() + (dd * 2) * NSTATE * NF + ab;
+      constexpr int CH = ff_chunk(N, (int)sizeof(TQ));
+      TQ *dst = FF + (k0 + e) * ff_per_element<N, TQ>() + (dd * 2) * CH + ab;
 #pragma unroll
       for (int s = 0; s < NSTATE; ++s) {
         dst[s * NF] = outN[s] * wt;                  // side 0: this element, its -d face
-        dst[(NSTATE + s) * NF] = outK[s] * wt;       // side 1: the -d neighbour, its +d face
+        dst[CH + s * NF] = outK[s] * wt;             // side 1: the -d neighbour, its +d face
       }
     }
   }
@@ -755,9 +769,11 @@ struct RhsLds {
   static constexpr int NN = N * N * N, NF = N * N;
   static constexpr int NV = NSTATE + NGRAD;          // 20 volume fields
   static constexpr int SV = NV * NN;
-  static constexpr int FE = 3 * 2 * NSTATE * NF;     // face fluxes of one element (3 directions, 2 sides)
+  static constexpr int CH = ff_chunk(N, (int)sizeof(T));
+  static constexpr int FE = 6 * CH;                  // face fluxes of one element (3 directions, 2 sides)
   static constexpr int SJ = NSTATE * NF;             // one face field set: the y or z jumps
-  static constexpr int BUF = SV + 2 * SJ;            // per element; holds FE first
+  static constexpr int VW = 16 / (int)sizeof(T);     // values per 16-byte word
+  static constexpr int BUF = (SV + 2 * SJ + VW - 1) / VW * VW;   // per element (16-byte multiple); holds FE first
   static_assert(FE <= SV, "face fluxes fit the volume buffer");
   // occupancy target (waves per SIMD) handed to the register allocator
   static constexpr int WAVES = sizeof(T) == 4 ? (N <= 5 ? 4 : 2) : (N <= 2 ? 4 : N <= 5 ? 3 : 2);
@@ -777,36 +793,33 @@ struct SOps {
   double wd[N];
 };
 
-// d = s f in type T (each coefficient rounded once)
+// The operators into LDS, one coefficient per thread (spread over the
+// waves): I, I^T, D^T and D / alpha_l (l = x, y, z) in type T, each
+// coefficient rounded once; the FP64 inverse mass; I L0 / alpha_l; the
+// Gauss weights.  a0..a2 = 1/alpha_l.
 template <int N, typename T>
-__device__ inline void scaled_symmat(SymMat<N, T> &d, const SymMat<N, double> &s, double f) {
-  using M = SymMat<N, T>;
-  for (int q = 0; q < M::QP; ++q) {
-    for (int j = 0; j < M::HP; ++j) { d.P[q][j] = T(s.P[q][j] * f); d.R[q][j] = T(s.R[q][j] * f); }
-    d.C[q] = T(s.C[q] * f);
-  }
-  for (int q = 0; q < M::NA; ++q)
-    for (int j = 0; j < M::NA; ++j) d.A[q][j] = T(s.A[q][j] * f);
-}
-
-template <int N, typename T>
-__device__ inline void load_sops(SOps<N, T> *so, const Ops<N, double> *op, const MeshArgs &m, int tid) {
-  if (tid == 0) convert_symmat(so->I, op->sI);
-  if (tid == 1) convert_symmat(so->It, op->sIt);
-  if (tid == 2) convert_symmat(so->Dt, op->sDt);
-  if (tid == 3) so->Mi = op->sMi;
-  if (tid == 4) scaled_symmat(so->Ds[0], op->sD, m.ialpha[0]);
-  if (tid == 5) scaled_symmat(so->Ds[1], op->sD, m.ialpha[1]);
-  if (tid == 6) scaled_symmat(so->Ds[2], op->sD, m.ialpha[2]);
-  if (tid >= 64 && tid < 64 + N) {
-    so->w[tid - 64] = T(op->w[tid - 64]);
-    so->wd[tid - 64] = op->w[tid - 64];
-  }
-  if (tid >= 128 && tid < 128 + N) {
-    const int q = tid - 128;
-    so->il0[0][q] = T(op->IL0[q] * m.ialpha[0]);
-    so->il0[1][q] = T(op->IL0[q] * m.ialpha[1]);
-    so->il0[2][q] = T(op->IL0[q] * m.ialpha[2]);
+__device__ inline void load_sops(SOps<N, T> *so, const Ops<N, double> *op, double a0, double a1, double a2, int tid) {
+  constexpr int MS = sizeof(SymMat<N, T>) / sizeof(T);   // coefficients per operator
+  static_assert(sizeof(SymMat<N, T>) == MS * sizeof(T) && sizeof(SymMat<N, double>) == MS * sizeof(double),
+                "operators are plain coefficient arrays");
+  using SO = SOps<N, T>;
+  static_assert(offsetof(SO, Ds) == 3 * sizeof(SymMat<N, T>), "I, I^T, D^T, D_x, D_y, D_z are contiguous");
+  constexpr int B1 = 6 * MS, B2 = B1 + MS, B3 = B2 + 3 * N, B4 = B3 + N;
+  static_assert(B4 + N <= NT, "one coefficient per thread");
+  if (tid < B1) {
+    const int o = tid / MS, i = tid - o * MS;
+    const SymMat<N, double> *src = o == 0 ? &op->sI : o == 1 ? &op->sIt : o == 2 ? &op->sDt : &op->sD;
+    const double f = o < 3 ? 1.0 : o == 3 ? a0 : o == 4 ? a1 : a2;
+    reinterpret_cast<T *>(&so->I)[tid] = T(reinterpret_cast<const double *>(src)[i] * f);
+  } else if (tid < B2) {
+    reinterpret_cast<double *>(&so->Mi)[tid - B1] = reinterpret_cast<const double *>(&op->sMi)[tid - B1];
+  } else if (tid < B3) {
+    const int l = (tid - B2) / N, q = tid - B2 - l * N;
+    so->il0[l][q] = T(op->IL0[q] * (l == 0 ? a0 : l == 1 ? a1 : a2));
+  } else if (tid < B4) {
+    so->w[tid - B3] = T(op->w[tid - B3]);
+  } else if (tid < B4 + N) {
+    so->wd[tid - B4] = op->w[tid - B4];
   }
 }
 
@@ -832,36 +845,43 @@ template <int N, int E, typename T>
 __device__ inline void rhs_faces(const T *__restrict__ FF, const long (*snb)[6], long k0, int ne,
                                  const SOps<N, T> *__restrict__ so, T (*__restrict__ sbuf)[RhsLds<N, T>::BUF],
                                  T (*__restrict__ sr)[NSTATE][N * N * N], int tid) {
-  constexpr int NF = N * N, NS = NSTATE * NF;
-  constexpr int FE = RhsLds<N, T>::FE;
-  constexpr int FL = (E * FE + NT - 1) / NT;
-  // loads of all directions first (a record is 2 NS contiguous values per
-  // direction: the own one for side 0, the +d neighbour's for side 1)
+  constexpr int NF = N * N;
+  constexpr int CH = RhsLds<N, T>::CH, FE = RhsLds<N, T>::FE;
+  using V = typename Vec16<T>::type;
+  constexpr int VC = CH * (int)sizeof(T) / 16;      // 16-byte words per chunk
+  constexpr int FL = (E * 6 * VC + NT - 1) / NT;
+  // 16-byte loads of the 6 chunks (direction, side) of every element, all
+  // issued first: side 0 from the element's own record, side 1 from the +d
+  // neighbour's
   {
-    T v[FL];
+    V v[FL];
 #pragma unroll
     for (int it = 0; it < FL; ++it) {
       const unsigned w = tid + it * NT;
-      const unsigned e = w / FE, r = w - e * FE;
-      const unsigned dd = r / (2 * NS), side = (r / NS) & 1;
-      const long K = side ? snb[e < (unsigned)ne ? e : 0][2 * dd + 1] : k0 + e;
-      v[it] = e < (unsigned)ne ? FF[K * FE + r] : T(0);
+      const unsigned ec = w / VC, j = w - ec * VC;  // ec = e * 6 + chunk
+      const unsigned e = ec / 6, c = ec - 6 * e;
+      const unsigned es = e < (unsigned)ne ? e : 0;
+      const long K = (c & 1) ? snb[es][c | 1] : k0 + es;   // c | 1 = 2 dd + 1: the +d neighbour
+      v[it] = reinterpret_cast<const V *>(FF + K * FE + c * CH)[j];
     }
 #pragma unroll
     for (int it = 0; it < FL; ++it) {
       const unsigned w = tid + it * NT;
-      const unsigned e = w / FE, r = w - e * FE;
-      if (e < (unsigned)ne) sbuf[e][r] = v[it];
+      const unsigned ec = w / VC, j = w - ec * VC;
+      const unsigned e = ec / 6;
+      if (e < (unsigned)ne) reinterpret_cast<V *>(&sbuf[e][(ec - 6 * e) * CH])[j] = v[it];
     }
   }
   __syncthreads();
-  // I^T along a, in place: lines of N values at (direction, side, state, qb)
+  // I^T along a, in place: lines of N values at (chunk, state, qb)
   {
+    constexpr int LC = NSTATE * N;                  // lines per chunk
     SymReg<N, T> It;
     It.load(so->It);
-    for (unsigned w = tid; w < (unsigned)ne * (FE / N); w += NT) {
-      const unsigned e = w / (FE / N), r = w - e * (FE / N);
-      T *p = &sbuf[e][N * r];
+    for (unsigned w = tid; w < (unsigned)ne * 6 * LC; w += NT) {
+      const unsigned ec = w / LC, r = w - ec * LC;
+      const unsigned e = ec / 6, c = ec - 6 * e;
+      T *p = &sbuf[e][c * CH + N * r];
       T v[N], o[N];
 #pragma unroll
       for (int q = 0; q < N; ++q) v[q] = p[q];
@@ -881,8 +901,9 @@ __device__ inline void rhs_faces(const T *__restrict__ FF, const long (*snb)[6],
     const int bst = dd == 2 ? N : N * N;      // memory stride of the face coordinate b
     for (unsigned w = tid; w < (unsigned)ne * 2 * NSTATE * N; w += NT) {
       const unsigned e = w / (2 * NSTATE * N), r = w - e * (2 * NSTATE * N);
-      const unsigned side = r / (NSTATE * N), s = (r / N) % NSTATE, a = r % N;
-      const T *p = &sbuf[e][(dd * 2 + side) * NS + s * NF + a];
+      const unsigned side = r / (NSTATE * N), sa = r - side * (NSTATE * N);
+      const unsigned s = sa / N, a = sa - s * N;
+      const T *p = &sbuf[e][(dd * 2 + side) * CH + s * NF + a];
       T v[N], o[N];
 #pragma unroll
       for (int q = 0; q < N; ++q) v[q] = p[q * N];
@@ -963,7 +984,7 @@ ldg_rhs_kernel(const UT *__restrict__ U, const T *__restrict__ FF, LaunchArgs la
   }
 
   fill_neighbors(m, ec, d.elem_off, ne, snb, tid);
-  load_sops<N, T>(&sops, opp, m, tid);
+  load_sops<N, T>(&sops, opp, m.ialpha[0], m.ialpha[1], m.ialpha[2], tid);
   for (int w = tid; w < ne * NSTATE * NN; w += NT) (&sr[0][0][0])[w] = T(0);
   __syncthreads();
   // the neighbours' traces the jumps need: the -x neighbour's node at the
