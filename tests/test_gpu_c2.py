@@ -7,8 +7,8 @@ tests/golden/make_c2_history.py):
   - the Gauss-point kinetic energy (1e-12) and enstrophy (1e-10) every 10
     steps (no reference counterpart);
   - the final state on 32 sampled elements, within 1e-12 relative L2 per
-    conserved variable (rho w: within 4 x the FP64 oracle's own rounding
-    spread at this horizon, measured by the fixture script's sibling test);
+    conserved variable, 1e-10 for rho w (zero in the initial state: its
+    norm is small, so its relative error is larger);
   - the lagged dissipation -dEk/dt (solver.cpp:437-440) within 1e-7
     relative where it is larger than 1e-4 of its maximum."""
 import os
@@ -29,7 +29,6 @@ def dissipation(ke, dt):
     return -(ke[2:] - ke[:-2]) / (2 * dt)
 
 
-@pytest.mark.skipif(not os.path.exists(FIXTURE), reason="fixture not generated yet (tests/golden/make_c2_history.py)")
 def test_c2_histories_match_oracle():
     f = np.load(FIXTURE)
     p, nx, nsteps, every = int(f["p"]), int(f["nx"]), int(f["nsteps"]), 10
