@@ -561,7 +561,7 @@ ldg_grad_kernel(const UT *__restrict__ U, TQ *__restrict__ Q, TQ *__restrict__ F
       const unsigned w = tid + it * NT < tot ? tid + it * NT : tot - 1;
       const unsigned e = w / NN;
       so[it] = w + e * ((NSTATE - 1) * NN);   // su[e][0][node]
-      const UT *src = U + (k0 + e) * (NSTATE * NN) + (w - e * NN);
+      const UT *src = U + k0 * (NSTATE * NN) + (e * (NSTATE * NN) + (w - e * NN));
 #pragma unroll
       for (int s = 0; s < NSTATE; ++s) v[it][s] = src[s * NN];
     }
@@ -745,7 +745,7 @@ ldg_grad_kernel(const UT *__restrict__ U, TQ *__restrict__ Q, TQ *__restrict__ F
       const double ia = dd == 0 ? m.ialpha[0] : dd == 1 ? m.ialpha[1] : m.ialpha[2];
       const C wt = sw[ab % N] * sw[ab / N] * C(ia);
       constexpr int CH = ff_chunk(N, (int)sizeof(TQ));
-      TQ *dst = FF + (k0 + e) * ff_per_element<N, TQ>() + (dd * 2) * CH + ab;
+      TQ *dst = FF + k0 * ff_per_element<N, TQ>() + (e * ff_per_element<N, TQ>() + (dd * 2) * CH + ab);
 #pragma unroll
       for (int s = 0; s < NSTATE; ++s) {
         dst[s * NF] = outN[s] * wt;                  // side 0: this element, its -d face
@@ -891,24 +891,43 @@ __device__ inline void rhs_faces(const T *__restrict__ FF, const long (*snb)[6],
     }
   }
   __syncthreads();
-  // I^T along b, subtracted into the face nodes of the residual
+  // I^T along b, subtracted into the face nodes of the residual, one
+  // direction at a time; a line is (e, side, s, a), the same for the three
+  // directions, so a thread decomposes its lines once
   SymReg<N, T> It;
   It.load(so->It);
+  constexpr int BL = 2 * NSTATE * N;              // lines per element and direction
+  constexpr int BI = (E * BL + NT - 1) / NT;
+  unsigned bsrc[BI], bdst[BI];                    // line start in sbuf[0] (chunk 0) and in sr[0], ~0 past the block
+  bool bside[BI];
+#pragma unroll
+  for (int it = 0; it < BI; ++it) {
+    const unsigned w = tid + it * NT;
+    const unsigned e = w / BL, r = w - e * BL;
+    const unsigned side = r / (NSTATE * N), sa = r - side * (NSTATE * N);
+    const unsigned s = sa / N, a = sa - s * N;
+    bside[it] = side != 0;
+    bsrc[it] = w < (unsigned)ne * BL ? e * RhsLds<N, T>::BUF + side * CH + s * NF + a : ~0u;
+    bdst[it] = (e * NSTATE + s) * (N * N * N);
+    // the face coordinate a of direction d has memory stride N (x faces) or 1
+    bdst[it] |= a << 24;
+  }
 #pragma unroll
   for (int dd = 0; dd < 3; ++dd) {
-    const int st_d = dd == 0 ? 1 : dd == 1 ? N : N * N;
+    constexpr int ST[3] = {1, N, N * N};
+    const int st_d = ST[dd];
     const int ast = dd == 0 ? N : 1;          // memory stride of the face coordinate a
     const int bst = dd == 2 ? N : N * N;      // memory stride of the face coordinate b
-    for (unsigned w = tid; w < (unsigned)ne * 2 * NSTATE * N; w += NT) {
-      const unsigned e = w / (2 * NSTATE * N), r = w - e * (2 * NSTATE * N);
-      const unsigned side = r / (NSTATE * N), sa = r - side * (NSTATE * N);
-      const unsigned s = sa / N, a = sa - s * N;
-      const T *p = &sbuf[e][(dd * 2 + side) * CH + s * NF + a];
+#pragma unroll
+    for (int it = 0; it < BI; ++it) {
+      if (bsrc[it] == ~0u) continue;
+      const T *p = &sbuf[0][bsrc[it] + dd * 2 * CH];
       T v[N], o[N];
 #pragma unroll
       for (int q = 0; q < N; ++q) v[q] = p[q * N];
       sym_apply<N, 1, T>(It.m, v, o);
-      T *dst = &sr[e][s][a * ast + (side ? (N - 1) * st_d : 0)];
+      const unsigned a = bdst[it] >> 24;
+      T *dst = &sr[0][0][(bdst[it] & 0xffffffu) + a * ast + (bside[it] ? (N - 1) * st_d : 0)];
 #pragma unroll
       for (int b = 0; b < N; ++b) dst[b * bst] -= o[b];
     }
@@ -925,6 +944,7 @@ ldg_rhs_kernel(const UT *__restrict__ U, const T *__restrict__ FF, LaunchArgs la
   __shared__ __align__(16) T sbuf[E][BUF];
   __shared__ T sr[E][NSTATE][NN];
   __shared__ double sred[NT / 64];
+  __shared__ double sbg[NSTATE];      // the background state (rho0, 0, 0, 0, E0)
   __shared__ SOps<N, T> sops;
   __shared__ long snb[E][6];
   const int lb = logical_block(la);
@@ -978,7 +998,7 @@ ldg_rhs_kernel(const UT *__restrict__ U, const T *__restrict__ FF, LaunchArgs la
     xi[it] = x_item(it);
     const unsigned i = xi[it] == ~0u ? totx - 1 : xi[it];
     const unsigned e = i / XL, r = i - e * XL;       // r = s NF + ab: the line starts at N r in its element
-    const UT *src = U + (k0 + e) * (NSTATE * NN) + N * r;
+    const UT *src = U + k0 * (NSTATE * NN) + (e * (NSTATE * NN) + N * r);
 #pragma unroll
     for (int q = 0; q < N; ++q) vo[it][q] = src[q];
   }
@@ -986,6 +1006,7 @@ ldg_rhs_kernel(const UT *__restrict__ U, const T *__restrict__ FF, LaunchArgs la
   fill_neighbors(m, ec, d.elem_off, ne, snb, tid);
   load_sops<N, T>(&sops, opp, m.ialpha[0], m.ialpha[1], m.ialpha[2], tid);
   for (int w = tid; w < ne * NSTATE * NN; w += NT) (&sr[0][0][0])[w] = T(0);
+  if (tid < NSTATE) sbg[tid] = tid == 0 ? ph.rho0 : tid == NSTATE - 1 ? ph.E0 : 0.0;
   __syncthreads();
   // the neighbours' traces the jumps need: the -x neighbour's node at the
   // line's start, and the -y (-z) neighbour's x-line along the -y (-z) face;
@@ -996,7 +1017,7 @@ ldg_rhs_kernel(const UT *__restrict__ U, const T *__restrict__ FF, LaunchArgs la
     const unsigned i = xi[it] == ~0u ? totx - 1 : xi[it];
     const unsigned e = i / XL, r = i - e * XL, s = r / NF, ab = r - s * NF;
     const unsigned iy = ab % N, iz = ab / N;
-    const UT *own = U + (k0 + e) * (NSTATE * NN) + N * r;
+    const UT *own = U + k0 * (NSTATE * NN) + (e * (NSTATE * NN) + N * r);
     vx[it] = U[snb[e][0] * (NSTATE * NN) + (N * r + (N - 1))];
     const UT *ny = iy == 0 ? U + snb[e][2] * (NSTATE * NN) + (s * NN + N * (N - 1) + NF * iz) : own;
     const UT *nz = iz == 0 ? U + snb[e][4] * (NSTATE * NN) + (s * NN + NF * (N - 1) + N * iy) : own;
@@ -1327,18 +1348,32 @@ ldg_rhs_kernel(const UT *__restrict__ U, const T *__restrict__ FF, LaunchArgs la
       }
     }
   }
+  // Epilogue items: the block's values are contiguous in every (node,
+  // state, element) array, so item w (< ne 5 N^3) is word gofs + w there
+  // (uniform base, 32-bit offset) and word li = e BUFD + r of the FP64
+  // view of the volume buffer (element e, r = w - 5 N^3 e).
   constexpr int EPI = (E * NSTATE * NN + NT - 1) / NT;
-  const int tot_epi = ne * NSTATE * NN;
+  constexpr unsigned BUFD = L::BUF * sizeof(T) / sizeof(double);   // an element's buffer, in doubles
+  const unsigned tot_epi = ne * NSTATE * NN;
   const long gofs = k0 * NSTATE * NN;
-  double ru[EPI], ra[EPI];
+  unsigned li[EPI];
 #pragma unroll
   for (int it = 0; it < EPI; ++it) {
-    const int w = tid + it * NT;
-    ru[it] = 0.0;
-    ra[it] = 0.0;
-    if (!PF && w < tot_epi) {
-      if (need_u0) ru[it] = st.u0[gofs + w];
-      if (need_acc) ra[it] = st.acc[gofs + w];
+    const unsigned w = tid + it * NT, e = w / (NSTATE * NN);
+    li[it] = w + e * (BUFD - NSTATE * NN);
+  }
+  double ru[EPI], ra[EPI];
+  {
+    const double *const u0b = st.u0 + gofs, *const accb = st.acc + gofs;
+#pragma unroll
+    for (int it = 0; it < EPI; ++it) {
+      const unsigned w = tid + it * NT;
+      ru[it] = 0.0;
+      ra[it] = 0.0;
+      if (!PF && w < tot_epi) {
+        if (need_u0) ru[it] = u0b[w];
+        if (need_acc) ra[it] = accb[w];
+      }
     }
   }
   // M1^{-1} along y, then along z (FP64)
@@ -1375,43 +1410,39 @@ ldg_rhs_kernel(const UT *__restrict__ U, const T *__restrict__ FF, LaunchArgs la
   // ---- epilogue (FP64): k, RK4 accumulation and next stage input.  All of
   // a thread's HBM reads are issued before any is used.
   {
-    const int tot = tot_epi;
+    const unsigned tot = tot_epi;
+    double *const mb = md(0);
     double kv[EPI];
 #pragma unroll
     for (int it = 0; it < EPI; ++it) {
-      const int w = tid + it * NT;
+      const unsigned w = tid + it * NT;
       kv[it] = 0.0;
       if (w < tot) {
-        const int e = w / (NSTATE * NN), r = w - e * (NSTATE * NN);
-        if (PF && need_u0) ru[it] = pf_u0(e)[r];
-        if (PF && need_acc) ra[it] = pf_acc(e)[r];
-        kv[it] = md(e)[r];
-        if (st.corr && st.gate[k0 + e]) kv[it] += st.corr[gofs + w];
+        if (PF && need_u0) ru[it] = mb[li[it] + NSTATE * NN];
+        if (PF && need_acc) ra[it] = mb[li[it] + 2 * NSTATE * NN];
+        kv[it] = mb[li[it]];
+        if (st.corr && st.gate[k0 + w / (NSTATE * NN)]) kv[it] += st.corr[gofs + w];
       }
     }
     if (st.ke_part) __syncthreads();   // every k read before the new state reuses the buffer
+    double *const koutb = st.kout + gofs, *const accb = st.acc + gofs, *const unextb = st.unext + gofs;
+    float *const next32b = st.next32 + gofs, *const unext32b = st.unext32 + gofs;
 #pragma unroll
     for (int it = 0; it < EPI; ++it) {
-      const int w = tid + it * NT;
+      const unsigned w = tid + it * NT;
       if (w < tot) {
-        const int e = w / (NSTATE * NN), r = w - e * (NSTATE * NN);
         const double k = kv[it];
-        const long gi = gofs + w;
-        if (st.kout) st.kout[gi] = k;
+        // background of the item's state variable (the FP32 copies are fluctuations)
+        const double bgv = sbg[(w / NN) % NSTATE];
+        if (st.kout) koutb[w] = k;
         if (st.acc) {
           const double a = (st.acc_init ? ru[it] : ra[it]) + dtb * k;
-          st.acc[gi] = a;
-          if (st.ke_part) md(e)[r] = a;
-          if (st.next32) {
-            const int sf = r / NN;
-            st.next32[gi] = (float)(a - (sf == 0 ? ph.rho0 : sf == 4 ? ph.E0 : 0.0));
-          }
+          accb[w] = a;
+          if (st.ke_part) mb[li[it]] = a;
+          if (st.next32) next32b[w] = (float)(a - bgv);
         }
-        if (st.unext) st.unext[gi] = ru[it] + dta * k;
-        if (st.unext32) {
-          const int sf = r / NN;
-          st.unext32[gi] = (float)(ru[it] + dta * k - (sf == 0 ? ph.rho0 : sf == 4 ? ph.E0 : 0.0));
-        }
+        if (st.unext) unextb[w] = ru[it] + dta * k;
+        if (st.unext32) unext32b[w] = (float)(ru[it] + dta * k - bgv);
       }
     }
   }
