@@ -79,25 +79,44 @@ def cpu_baseline(p, ne, steps, ne1, steps1):
                                      f"{wall1:.1f} s"}}
 
 
-def alg_bytes(dof, mixed):
-    """Algorithmic HBM bytes per launch, averaged over the 4 RK stages of a
-    step in a long run.  u^n and the running sum are 8-byte words, Q is 3
-    fields per state variable (8 or, mixed, 4 bytes), a stage input U_s is
-    8 bytes (FP64) or a 4-byte fluctuation (mixed: stage 1 reads the copy of
-    u^n that the previous step's stage 4 wrote).
-      grad: read U_s, write Q
-      rhs : read U_s, Q and u^n; read (stages 2-4) and write the running
+def ff_bytes_per_dof(p, mixed):
+    """Face-flux array bytes per DOF: per element 3 directions x 2 sides,
+    each a chunk of 5 (p+1)^2 values padded to 16 bytes."""
+    n, w = p + 1, (4 if mixed else 8)
+    chunk = -(-5 * n * n * w // 16) * 16 // w
+    return 6 * chunk * w / (5 * n ** 3)
+
+
+def alg_bytes(dof, mixed, p=4):
+    """Algorithmic HBM bytes per launch of this design, averaged over the 4
+    RK stages of a step in a long run: every array a kernel must read or
+    write, counted once at its storage precision.  u^n and the running sum
+    are 8-byte words; a stage input U_s is 8 bytes (FP64) or a 4-byte
+    fluctuation (mixed: stage 1 reads the copy of u^n that the previous
+    step's stage 4 wrote); FF is the face-flux array (ff_bytes_per_dof).
+    The gradient never reaches HBM: the gradient kernel evaluates it on the
+    faces, the rhs kernel at the volume Gauss points.
+      grad: read U_s, write FF
+      rhs : read U_s, FF and u^n; read (stages 2-4) and write the running
             sum; write U_{s+1} (stages 1-3) or, mixed, the FP32 copy of
             u^{n+1} (stage 4).  In the FP64 path stage 1's U_s is u^n
             itself, read once."""
-    q = 3 * (4 if mixed else 8)
+    ff = ff_bytes_per_dof(p, mixed)
     us = 4 if mixed else 8
-    grad = (us + q) * dof
-    s1 = (us + q + 8 if mixed else 8 + q) + 8 + us          # U_s, Q, (u^n), acc write, U_2
-    s23 = us + q + 8 + 16 + us                              # U_s, Q, u^n, acc read+write, U_{s+1}
-    s4 = us + q + 8 + 16 + (4 if mixed else 0)              # U_4, Q, u^n, acc read+write, (copy of u^{n+1})
+    grad = (us + ff) * dof
+    s1 = (us + 8 if mixed else 8) + ff + 8 + us             # U_s, (u^n), FF, acc write, U_2
+    s23 = us + ff + 8 + 16 + us                             # U_s, FF, u^n, acc read+write, U_{s+1}
+    s4 = us + ff + 8 + 16 + (4 if mixed else 0)             # U_4, FF, u^n, acc read+write, (copy of u^{n+1})
     rhs = (s1 + 2 * s23 + s4) / 4 * dof
     return grad, rhs
+
+
+def survey_bytes_per_dof_stage(mixed):
+    """The two-kernel model's compulsory bytes per DOF-stage in which the
+    nodal gradient Q makes an HBM round trip (read U_s and write Q; read
+    U_s, Q, u^n, read+write the running sum, write U_{s+1}): 12 FP64 words
+    = 96 B, or 72 B with Q in FP32."""
+    return 72.0 if mixed else 96.0
 
 
 # the device code the committed rocprofv3 figures were measured on
@@ -377,8 +396,9 @@ def main():
     value = 4 * args.steps * dof / r["wall"]
     fp64_value = 4 * args.steps * dof / runs["fp64"]["wall"]
     mixed = main_name == "mixed"
-    b_grad, b_rhs = alg_bytes(dof, mixed)
+    b_grad, b_rhs = alg_bytes(dof, mixed, p)
     ach_rhs = b_rhs / (r["kt"]["rhs_ms"] * 1e-3) / 1e9
+    stage_ms = r["kt"]["grad_ms"] + r["kt"]["rhs_ms"]
     err = rel_l2(runs["mixed"]["u"], runs["fp64"]["u"])
 
     out = {
@@ -408,13 +428,18 @@ def main():
         "roofline": {"bound": "hbm", "achieved": ach_rhs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": ach_rhs / HBM_PEAK_GBS, "traffic": profiled_traffic("rhs", main_name, p, nx),
                      "traffic_source": "profiles/ldg_kernels_tgv64_p4.md (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0)",
-                     "kernel": "ldg_rhs_kernel", "alg_bytes_per_launch": b_rhs, "avg_ms": r["kt"]["rhs_ms"]},
+                     "kernel": "ldg_rhs_kernel", "alg_bytes_per_launch": b_rhs, "avg_ms": r["kt"]["rhs_ms"],
+                     "two_kernel_model": {"bytes_per_dof_stage": survey_bytes_per_dof_stage(mixed),
+                                          "effective_GBs": survey_bytes_per_dof_stage(mixed) * dof
+                                          / (stage_ms * 1e-3) / 1e9,
+                                          "note": "the bytes a stage would move with the nodal gradient in "
+                                                  "HBM, over the measured grad + rhs time of a stage"}},
         "issue_bound": issue_bound("rhs", main_name, p, nx, r["kt"]["rhs_ms"]),
         "kernels": {
             name: {"ldg_grad_kernel": {"avg_ms": v["kt"]["grad_ms"],
-                                       "alg_GBs": alg_bytes(dof, name == "mixed")[0] / (v["kt"]["grad_ms"] * 1e-3) / 1e9},
+                                       "alg_GBs": alg_bytes(dof, name == "mixed", p)[0] / (v["kt"]["grad_ms"] * 1e-3) / 1e9},
                    "ldg_rhs_kernel": {"avg_ms": v["kt"]["rhs_ms"],
-                                      "alg_GBs": alg_bytes(dof, name == "mixed")[1] / (v["kt"]["rhs_ms"] * 1e-3) / 1e9},
+                                      "alg_GBs": alg_bytes(dof, name == "mixed", p)[1] / (v["kt"]["rhs_ms"] * 1e-3) / 1e9},
                    "ms_per_step": v["wall"] / args.steps * 1e3,
                    "tiling": {"p": v["tiling"][0][0], "grad_elements_per_workgroup": v["tiling"][0][1],
                               "rhs_elements_per_workgroup": v["tiling"][0][2]} if v["tiling"] else "default"}

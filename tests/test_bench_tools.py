@@ -11,10 +11,19 @@ import bench  # noqa: E402
 
 
 def test_alg_bytes_per_dof():
-    # FP64: grad U 8 + Q 24; rhs (48 + 2 x 64 + 56) / 4 = 58 B.  Mixed: FP32
-    # stage inputs and gradient; rhs (36 + 2 x 44 + 44) / 4 = 42 B.
-    assert bench.alg_bytes(1, False) == (32.0, 58.0)
-    assert bench.alg_bytes(1, True) == (16.0, 42.0)
+    # p = 4: an element has 625 DOF and 6 face-flux chunks of 5 x 25 values,
+    # padded to 16 bytes: 126 doubles or 128 floats.
+    ff64, ff32 = 6 * 126 * 8 / 625, 6 * 128 * 4 / 625
+    assert abs(bench.ff_bytes_per_dof(4, False) - ff64) < 1e-12 and abs(bench.ff_bytes_per_dof(4, True) - ff32) < 1e-12
+    # FP64: grad U 8 + FF; rhs stages (u^n once, acc write, U_2), 2 x (U, u^n,
+    # acc r+w, U_next), (U, u^n, acc r+w), each + FF
+    g, r = bench.alg_bytes(1, False)
+    assert abs(g - (8 + ff64)) < 1e-12 and abs(r - ((24 + 2 * 40 + 32) / 4 + ff64)) < 1e-12
+    # mixed: 4-byte stage inputs; stage 1 reads u^n beside its FP32 copy,
+    # stage 4 writes the FP32 copy of u^{n+1}
+    g, r = bench.alg_bytes(1, True)
+    assert abs(g - (4 + ff32)) < 1e-12 and abs(r - ((24 + 2 * 32 + 32) / 4 + ff32)) < 1e-12
+    assert bench.survey_bytes_per_dof_stage(False) == 96.0 and bench.survey_bytes_per_dof_stage(True) == 72.0
 
 
 def _fingerprint(path, sample, ke, norms, dt=1e-3, steps=None, idx=None):
