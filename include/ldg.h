@@ -119,7 +119,9 @@ int ldg_step(ldg_handle *h, double dt, int nsteps, double *ke);
 /* Diagnostics of the resident state: out[0] = reference kinetic energy
    (Solver::computeKE, solver.cpp:1007-1034); out[1] = kinetic energy and
    out[2] = enstrophy by Gauss quadrature of the interpolated state and LDG
-   gradient (no reference counterpart). */
+   gradient (no reference counterpart).  There is no separate kinetic-energy
+   or enstrophy entry point: the reference's per-step KE record (kes,
+   solver.cpp:437) is ldg_step's ke output, and this call covers the rest. */
 int ldg_diagnostics(ldg_handle *h, double out[3]);
 
 /* Average device time per launch of each kernel over nsteps RK4 steps of
