@@ -988,6 +988,41 @@ ldg_rhs_kernel(const UT *__restrict__ U, const T *__restrict__ FF, LaunchArgs la
   constexpr int XI = XSPLIT ? 1 : (int)((KP + NT - 1) / NT);
   const unsigned totx = ne * XL;
   UT vo[XI][N], vx[XI], vy[XI][N], vz[XI][N];
+  // Line items (e, s, ab) of the sweep phases below run in kinds of KP
+  // items (a kind = one family of lines, e.g. the z-lines of F_x).  When KP
+  // divides the workgroup (p = 4: E = 2 fills it, E = 1 half of it) a
+  // thread's item is the same in every pass and phase: it is decomposed once
+  // here, and kind_loop gives the thread the kinds (pass * NT / KP +
+  // tid / KP) of its wave-uniform share.
+  constexpr bool HOIST = KP <= NT && NT % KP == 0;
+  struct LineIt {
+    unsigned e, s, ab;
+  };
+  auto line_at = [&](unsigned i) -> LineIt {
+    LineIt L;
+    L.e = i / XL;
+    const unsigned r = i - L.e * XL;
+    L.s = r / NF;
+    L.ab = r - L.s * NF;
+    return L;
+  };
+  const unsigned own_i = HOIST ? (unsigned)tid % KP : 0u;
+  const LineIt own = line_at(own_i < totx ? own_i : 0u);
+  auto kind_loop = [&](const unsigned K, auto &&body) {
+    if constexpr (HOIST) {
+      constexpr unsigned PER = NT / KP;
+#pragma unroll
+      for (unsigned pass = 0; pass < (K + PER - 1) / PER; ++pass) {
+        const unsigned kind = pass * PER + (unsigned)tid / KP;
+        if (kind < K && own_i < totx) body(kind, own);
+      }
+    } else {
+      for (unsigned w = tid; w < K * KP; w += NT) {
+        const unsigned kind = w / KP, i = w - kind * KP;
+        if (i < totx) body(kind, line_at(i));
+      }
+    }
+  };
   unsigned xi[XI];                                   // the item, or ~0
   auto x_item = [&](int it) -> unsigned {
     const unsigned w = tid + it * NT, i = XSPLIT ? w % KP : w;
@@ -1111,12 +1146,9 @@ ldg_rhs_kernel(const UT *__restrict__ U, const T *__restrict__ FF, LaunchArgs la
     T l0[N];
 #pragma unroll
     for (int q = 0; q < N; ++q) l0[q] = so->il0[1][q];
-    for (unsigned w = tid; w < 2 * KP; w += NT) {
-      const unsigned kind = w / KP, i = w - kind * KP;
-      if (i >= totx) continue;
-      const unsigned e = i / XL, r = i - e * XL, s = r / NF, ab = r - s * NF;
-      const unsigned a = ab % N, b = ab / N;
-      T *p = &sbuf[e][(kind == 0 ? s : NSTATE + s) * NN + a + NF * b];
+    kind_loop(2, [&](unsigned kind, const LineIt &L) {
+      const unsigned a = L.ab % N, b = L.ab / N;
+      T *p = &sbuf[L.e][(kind == 0 ? L.s : NSTATE + L.s) * NN + a + NF * b];
       T v[N], o[N];
 #pragma unroll
       for (int q = 0; q < N; ++q) v[q] = p[N * q];
@@ -1124,13 +1156,13 @@ ldg_rhs_kernel(const UT *__restrict__ U, const T *__restrict__ FF, LaunchArgs la
 #pragma unroll
       for (int q = 0; q < N; ++q) p[N * q] = o[q];
       if (kind == 0) {
-        const T jy = sjy(e)[s * NF + ab];
+        const T jy = sjy(L.e)[L.s * NF + L.ab];
         sym_apply<N, -1, T>(Dy.m, v, o);
         T *pq = p + 2 * NSTATE * NN;
 #pragma unroll
         for (int q = 0; q < N; ++q) pq[N * q] = o[q] + l0[q] * jy;
       }
-    }
+    });
     for (unsigned w = tid; w < (unsigned)ne * NSTATE * N; w += NT) {
       const unsigned e = w / (NSTATE * N), r = w - e * (NSTATE * N);
       T *p = sjz(e) + (r / N) * NF + r % N;
@@ -1152,11 +1184,8 @@ ldg_rhs_kernel(const UT *__restrict__ U, const T *__restrict__ FF, LaunchArgs la
     T l0[N];
 #pragma unroll
     for (int q = 0; q < N; ++q) l0[q] = so->il0[2][q];
-    for (unsigned w = tid; w < 3 * KP; w += NT) {
-      const unsigned kind = w / KP, i = w - kind * KP;
-      if (i >= totx) continue;
-      const unsigned e = i / XL, r = i - e * XL, s = r / NF, ab = r - s * NF;
-      T *p = &sbuf[e][(kind * NSTATE + s) * NN + ab];
+    kind_loop(3, [&](unsigned kind, const LineIt &L) {
+      T *p = &sbuf[L.e][(kind * NSTATE + L.s) * NN + L.ab];
       T v[N], o[N];
 #pragma unroll
       for (int q = 0; q < N; ++q) v[q] = p[NF * q];
@@ -1164,13 +1193,13 @@ ldg_rhs_kernel(const UT *__restrict__ U, const T *__restrict__ FF, LaunchArgs la
 #pragma unroll
       for (int q = 0; q < N; ++q) p[NF * q] = o[q];
       if (kind == 0) {
-        const T jz = sjz(e)[s * NF + ab];
+        const T jz = sjz(L.e)[L.s * NF + L.ab];
         sym_apply<N, -1, T>(Dz.m, v, o);
         T *pq = p + 3 * NSTATE * NN;
 #pragma unroll
         for (int q = 0; q < N; ++q) pq[NF * q] = o[q] + l0[q] * jz;
       }
-    }
+    });
   }
   __syncthreads();
 
@@ -1214,51 +1243,33 @@ ldg_rhs_kernel(const UT *__restrict__ U, const T *__restrict__ FF, LaunchArgs la
   }
   __syncthreads();
 
-  // test-function contraction: z sweep (I^T on F_x, F_y; D^T on F_z)
+  // test-function contraction: z sweep (kinds 0, 1: I^T on F_x, F_y; kind
+  // 2: D^T on F_z)
   {
-    SymReg<N, T> It;
+    SymReg<N, T> It, Dt;
     It.load(so->It);
-    for (int w = tid; w < ne * 10 * NF; w += NT) {
-      const int e = w / (10 * NF);
-      const int r = w - e * (10 * NF);
-      T *p = &sbuf[e][NSTATE * NN + (r / NF) * NN + r % NF];
-      T v[N], o[N];
-#pragma unroll
-      for (int q = 0; q < N; ++q) v[q] = p[q * NF];
-      sym_apply<N, 1, T>(It.m, v, o);
-#pragma unroll
-      for (int i = 0; i < N; ++i) p[i * NF] = o[i];
-    }
-  }
-  {
-    SymReg<N, T> Dt;
     Dt.load(so->Dt);
-    for (int w = tid; w < ne * NSTATE * NF; w += NT) {
-      const int e = w / (NSTATE * NF);
-      const int r = w - e * (NSTATE * NF);
-      T *p = &sbuf[e][(NSTATE + 10) * NN + (r / NF) * NN + r % NF];
+    kind_loop(3, [&](unsigned kind, const LineIt &L) {
+      T *p = &sbuf[L.e][(NSTATE + kind * NSTATE + L.s) * NN + L.ab];
       T v[N], o[N];
 #pragma unroll
       for (int q = 0; q < N; ++q) v[q] = p[q * NF];
-      sym_apply<N, -1, T>(Dt.m, v, o);
+      if (kind < 2) sym_apply<N, 1, T>(It.m, v, o);
+      else sym_apply<N, -1, T>(Dt.m, v, o);
 #pragma unroll
       for (int i = 0; i < N; ++i) p[i * NF] = o[i];
-    }
+    });
   }
   __syncthreads();
   // y sweep: F_x <- I^T F_x ; G <- D^T F_y + I^T F_z (stored in the F_y slot)
   {
     SymReg<N, T> It;
     It.load(so->It);
-    for (int w = tid; w < ne * NSTATE * NF; w += NT) {
-      const int e = w / (NSTATE * NF);
-      const int r = w - e * (NSTATE * NF);
-      const int s = r / NF;
-      const int ab = r % NF;
-      const int base = (ab % N) + NF * (ab / N);
-      T *px = &sbuf[e][(NSTATE + s) * NN + base];
-      T *py = &sbuf[e][(NSTATE + 5 + s) * NN + base];
-      const T *pz = &sbuf[e][(NSTATE + 10 + s) * NN + base];
+    kind_loop(1, [&](unsigned, const LineIt &L) {
+      const unsigned base = (L.ab % N) + NF * (L.ab / N);
+      T *px = &sbuf[L.e][(NSTATE + L.s) * NN + base];
+      T *py = px + NSTATE * NN;
+      const T *pz = px + 2 * NSTATE * NN;
       T vx[N], vy[N], vz[N], ox[N], og[N];
 #pragma unroll
       for (int q = 0; q < N; ++q) { vx[q] = px[q * N]; vy[q] = py[q * N]; vz[q] = pz[q * N]; }
@@ -1269,7 +1280,7 @@ ldg_rhs_kernel(const UT *__restrict__ U, const T *__restrict__ FF, LaunchArgs la
         px[i * N] = ox[i];
         py[i * N] = og[i];
       }
-    }
+    });
   }
   __syncthreads();
   // x sweep: r = D^T F_x + I^T G + face residual (T), then M1^{-1} along x
@@ -1278,42 +1289,43 @@ ldg_rhs_kernel(const UT *__restrict__ U, const T *__restrict__ FF, LaunchArgs la
   // threads when T = float: every thread keeps its results in registers
   // until all reads are done.
   {
-    constexpr int XI = (E * NSTATE * NF + NT - 1) / NT;
-    const int tot = ne * NSTATE * NF;
-    double res[XI][N];
+    constexpr int XR = HOIST ? 1 : (E * XL + NT - 1) / NT;
+    double res[XR][N];
+    bool has[XR];
     SymReg<N, T> It;
     It.load(so->It);
+    auto xline = [&](const LineIt &L, double (&out)[N]) {
+      const unsigned base = N * L.ab;
+      const T *px = &sbuf[L.e][(NSTATE + L.s) * NN + base];
+      const T *pg = px + NSTATE * NN;
+      T vx[N], vg[N], o[N];
+      double rr[N];
 #pragma unroll
-    for (int it = 0; it < XI; ++it) {
-      const int w = tid + it * NT;
-      if (w < tot) {
-        const int e = w / (NSTATE * NF);
-        const int r = w - e * (NSTATE * NF);
-        const int s = r / NF;
-        const int base = N * (r % NF);
-        const T *px = &sbuf[e][(NSTATE + s) * NN + base];
-        const T *pg = &sbuf[e][(NSTATE + 5 + s) * NN + base];
-        T vx[N], vg[N], o[N];
-        double rr[N];
+      for (int q = 0; q < N; ++q) { vx[q] = px[q]; vg[q] = pg[q]; }
+      sym_apply2<N, -1, 1, T>(so->Dt, vx, It.m, vg, o);
 #pragma unroll
-        for (int q = 0; q < N; ++q) { vx[q] = px[q]; vg[q] = pg[q]; }
-        sym_apply2<N, -1, 1, T>(so->Dt, vx, It.m, vg, o);
+      for (int i = 0; i < N; ++i) rr[i] = (double)(sr[L.e][L.s][base + i] + o[i]);
+      sym_apply<N, 1, double>(so->Mi, rr, out);
+    };
+    if constexpr (HOIST) {
+      has[0] = tid < (int)KP && own_i < totx;
+      if (has[0]) xline(own, res[0]);
+    } else {
 #pragma unroll
-        for (int i = 0; i < N; ++i) rr[i] = (double)(sr[e][s][base + i] + o[i]);
-        sym_apply<N, 1, double>(so->Mi, rr, res[it]);
+      for (int it = 0; it < XR; ++it) {
+        const unsigned w = tid + it * NT;
+        has[it] = w < totx;
+        if (has[it]) xline(line_at(w), res[it]);
       }
     }
     __syncthreads();
 #pragma unroll
-    for (int it = 0; it < XI; ++it) {
-      const int w = tid + it * NT;
-      if (w < tot) {
-        const int e = w / (NSTATE * NF);
-        const int r = w - e * (NSTATE * NF);
-        double *po = md(e) + (r / NF) * NN + N * (r % NF);
+    for (int it = 0; it < XR; ++it) {
+      if (!has[it]) continue;
+      const LineIt L = HOIST ? own : line_at(tid + it * NT);
+      double *po = md(L.e) + L.s * NN + N * L.ab;
 #pragma unroll
-        for (int i = 0; i < N; ++i) po[i] = res[it][i];
-      }
+      for (int i = 0; i < N; ++i) po[i] = res[it][i];
     }
   }
   __syncthreads();
@@ -1382,20 +1394,17 @@ ldg_rhs_kernel(const UT *__restrict__ U, const T *__restrict__ FF, LaunchArgs la
     {
       SymReg<N, double> Mi;
       Mi.load(so->Mi);
-      for (int w = tid; w < ne * NSTATE * NF; w += NT) {
-        const int e = w / (NSTATE * NF);
-        const int r = w - e * (NSTATE * NF);
-        const int s = r / NF;
-        const int ab = r % NF;
-        const int ls = line_stride<N>(dd);
-        double *p = md(e) + s * NN + line_base<N>(dd, ab % N, ab / N);
+      kind_loop(1, [&](unsigned, const LineIt &L) {
+        constexpr unsigned LS[3] = {1, N, N * N};
+        const unsigned a = L.ab % N, b = L.ab / N;
+        double *p = md(L.e) + L.s * NN + (dd == 1 ? a + NF * b : a + N * b);
         double v[N], o[N];
 #pragma unroll
-        for (int i = 0; i < N; ++i) v[i] = p[i * ls];
+        for (int i = 0; i < N; ++i) v[i] = p[i * LS[dd]];
         sym_apply<N, 1, double>(Mi.m, v, o);
 #pragma unroll
-        for (int i = 0; i < N; ++i) p[i * ls] = o[i];
-      }
+        for (int i = 0; i < N; ++i) p[i * LS[dd]] = o[i];
+      });
     }
     __syncthreads();
   }
