@@ -421,9 +421,6 @@ int create_impl(const ldg_config *cfg, const ldg_instance *insts, int n, ldg_han
         return bad(LDG_EDEVICE);
       maxblk = std::max(maxblk, g.rhs[c].nblocks);
     }
-    // measured default: the FP64 rhs kernel at p = 4 is fastest with one
-    // element per workgroup (64^3 elements: 2.96 against 3.20 ms with two)
-    if (!h->mixed && g.n == 5) g.crhs = 1;
     std::vector<double> sc;
     for (int idx : g.members) sc.push_back(ke_scale(h->cfg, h->inst[idx]));
     if (hipMalloc((void **)&g.ke_part, sizeof(double) * (maxblk + 1)) != hipSuccess ||

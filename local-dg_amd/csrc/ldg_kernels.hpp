@@ -773,7 +773,14 @@ struct RhsLds {
   static constexpr int FE = 6 * CH;                  // face fluxes of one element (3 directions, 2 sides)
   static constexpr int SJ = NSTATE * NF;             // one face field set: the y or z jumps
   static constexpr int VW = 16 / (int)sizeof(T);     // values per 16-byte word
-  static constexpr int BUF = (SV + 2 * SJ + VW - 1) / VW * VW;   // per element (16-byte multiple); holds FE first
+  // the y and z jump sets: in FP64 inside the Q_z slot (which the volume
+  // stage fills last; less LDS, so more workgroups fit a CU), in FP32 after
+  // the volume fields (LDS does not limit the FP32 kernel's occupancy, and
+  // the Q_z slot would cost a barrier)
+  static constexpr bool JQZ = sizeof(T) == sizeof(double);
+  static_assert(2 * SJ <= NSTATE * NN, "the jump sets fit the Q_z slot");
+  static constexpr int SJ0 = JQZ ? 3 * NSTATE * NN : SV;   // start of the jump sets
+  static constexpr int BUF = ((JQZ ? SV : SV + 2 * SJ) + VW - 1) / VW * VW;   // per element (16-byte multiple); holds FE first
   static_assert(FE <= SV, "face fluxes fit the volume buffer");
   // occupancy target (waves per SIMD) handed to the register allocator
   static constexpr int WAVES = sizeof(T) == 4 ? (N <= 5 ? 4 : 2) : (N <= 2 ? 4 : N <= 5 ? 3 : 2);
@@ -1008,18 +1015,22 @@ ldg_rhs_kernel(const UT *__restrict__ U, const T *__restrict__ FF, LaunchArgs la
   };
   const unsigned own_i = HOIST ? (unsigned)tid % KP : 0u;
   const LineIt own = line_at(own_i < totx ? own_i : 0u);
+  // body(kind, item, pass): pass = the loop pass the item came in (kind-0
+  // items come in passes 0, 1, ... in order)
   auto kind_loop = [&](const unsigned K, auto &&body) {
     if constexpr (HOIST) {
       constexpr unsigned PER = NT / KP;
 #pragma unroll
       for (unsigned pass = 0; pass < (K + PER - 1) / PER; ++pass) {
         const unsigned kind = pass * PER + (unsigned)tid / KP;
-        if (kind < K && own_i < totx) body(kind, own);
+        if (kind < K && own_i < totx) body(kind, own, pass);
       }
     } else {
-      for (unsigned w = tid; w < K * KP; w += NT) {
+#pragma unroll
+      for (unsigned pass = 0; pass < (K * KP + NT - 1) / NT; ++pass) {
+        const unsigned w = tid + pass * NT;
         const unsigned kind = w / KP, i = w - kind * KP;
-        if (i < totx) body(kind, line_at(i));
+        if (kind < K && i < totx) body(kind, line_at(i), pass);
       }
     }
   };
@@ -1085,8 +1096,8 @@ ldg_rhs_kernel(const UT *__restrict__ U, const T *__restrict__ FF, LaunchArgs la
   // (y, z) = (a, b), the y-line (x, z) = (a, b) and the z-line (x, y) =
   // (a, b) of field s; each stage runs its kinds of lines in wave-aligned
   // ranges of KP items.
-  auto sjy = [&](int e) -> T * { return &sbuf[e][L::SV]; };
-  auto sjz = [&](int e) -> T * { return &sbuf[e][L::SV + L::SJ]; };
+  auto sjy = [&](int e) -> T * { return &sbuf[e][L::SJ0]; };
+  auto sjz = [&](int e) -> T * { return &sbuf[e][L::SJ0 + L::SJ]; };
   {
     SymReg<N, T> I, Dx;
     I.load(so->I);
@@ -1146,7 +1157,7 @@ ldg_rhs_kernel(const UT *__restrict__ U, const T *__restrict__ FF, LaunchArgs la
     T l0[N];
 #pragma unroll
     for (int q = 0; q < N; ++q) l0[q] = so->il0[1][q];
-    kind_loop(2, [&](unsigned kind, const LineIt &L) {
+    kind_loop(2, [&](unsigned kind, const LineIt &L, unsigned pass) {
       const unsigned a = L.ab % N, b = L.ab / N;
       T *p = &sbuf[L.e][(kind == 0 ? L.s : NSTATE + L.s) * NN + a + NF * b];
       T v[N], o[N];
@@ -1176,7 +1187,9 @@ ldg_rhs_kernel(const UT *__restrict__ U, const T *__restrict__ FF, LaunchArgs la
   }
   __syncthreads();
   // z sweeps: kind 0 I XY (in place) and D XY + IL0 J_z into the Q_z slot;
-  // kinds 1 and 2 I Q_x and I Q_y (in place)
+  // kinds 1 and 2 I Q_x and I Q_y (in place).  Where the z jumps share the
+  // Q_z slot (FP64), each kind-0 item reads its jump before a barrier that
+  // precedes the first Q_z store.
   {
     SymReg<N, T> I, Dz;
     I.load(so->I);
@@ -1184,7 +1197,16 @@ ldg_rhs_kernel(const UT *__restrict__ U, const T *__restrict__ FF, LaunchArgs la
     T l0[N];
 #pragma unroll
     for (int q = 0; q < N; ++q) l0[q] = so->il0[2][q];
-    kind_loop(3, [&](unsigned kind, const LineIt &L) {
+    constexpr int ZI = HOIST ? 1 : (int)((KP + NT - 1) / NT);   // kind-0 items per thread
+    T jzr[ZI];
+#pragma unroll
+    for (int it = 0; it < ZI; ++it) {
+      const unsigned i = HOIST ? own_i : tid + it * NT;
+      const LineIt L = HOIST ? own : line_at(i < totx ? i : 0u);
+      jzr[it] = (HOIST ? (unsigned)tid < KP : i < KP) && i < totx ? sjz(L.e)[L.s * NF + L.ab] : T(0);
+    }
+    if constexpr (L::JQZ) __syncthreads();
+    kind_loop(3, [&](unsigned kind, const LineIt &L, unsigned pass) {
       T *p = &sbuf[L.e][(kind * NSTATE + L.s) * NN + L.ab];
       T v[N], o[N];
 #pragma unroll
@@ -1193,7 +1215,8 @@ ldg_rhs_kernel(const UT *__restrict__ U, const T *__restrict__ FF, LaunchArgs la
 #pragma unroll
       for (int q = 0; q < N; ++q) p[NF * q] = o[q];
       if (kind == 0) {
-        const T jz = sjz(L.e)[L.s * NF + L.ab];
+        // this item's jump (kind-0 items come in passes 0 .. ZI-1)
+        const T jz = jzr[HOIST ? 0 : pass];
         sym_apply<N, -1, T>(Dz.m, v, o);
         T *pq = p + 3 * NSTATE * NN;
 #pragma unroll
@@ -1249,7 +1272,7 @@ ldg_rhs_kernel(const UT *__restrict__ U, const T *__restrict__ FF, LaunchArgs la
     SymReg<N, T> It, Dt;
     It.load(so->It);
     Dt.load(so->Dt);
-    kind_loop(3, [&](unsigned kind, const LineIt &L) {
+    kind_loop(3, [&](unsigned kind, const LineIt &L, unsigned pass) {
       T *p = &sbuf[L.e][(NSTATE + kind * NSTATE + L.s) * NN + L.ab];
       T v[N], o[N];
 #pragma unroll
@@ -1265,7 +1288,7 @@ ldg_rhs_kernel(const UT *__restrict__ U, const T *__restrict__ FF, LaunchArgs la
   {
     SymReg<N, T> It;
     It.load(so->It);
-    kind_loop(1, [&](unsigned, const LineIt &L) {
+    kind_loop(1, [&](unsigned, const LineIt &L, unsigned) {
       const unsigned base = (L.ab % N) + NF * (L.ab / N);
       T *px = &sbuf[L.e][(NSTATE + L.s) * NN + base];
       T *py = px + NSTATE * NN;
@@ -1394,7 +1417,7 @@ ldg_rhs_kernel(const UT *__restrict__ U, const T *__restrict__ FF, LaunchArgs la
     {
       SymReg<N, double> Mi;
       Mi.load(so->Mi);
-      kind_loop(1, [&](unsigned, const LineIt &L) {
+      kind_loop(1, [&](unsigned, const LineIt &L, unsigned) {
         constexpr unsigned LS[3] = {1, N, N * N};
         const unsigned a = L.ab % N, b = L.ab / N;
         double *p = md(L.e) + L.s * NN + (dd == 1 ? a + NF * b : a + N * b);
@@ -1481,21 +1504,28 @@ static __global__ void __launch_bounds__(NT) ldg_finalize_kernel(const double *_
   const int i = blockIdx.x;
   const int b0 = inst[i].block_off, b1 = b0 + inst[i].nblocks;
   const long row = step ? *step : 0;
+  constexpr int A = 32;   // accumulators per thread: loads in flight (one workgroup reads every partial)
   for (int j = 0; j < stride; ++j) {
-    // thread t sums blocks b0 + t + k NT into accumulator k mod 8 (eight
-    // loads in flight; the order depends only on b - b0)
-    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (int base = b0 + threadIdx.x; base < b1; base += 8 * NT) {
-      double x[8];
+    // thread t sums blocks b0 + t + k NT into accumulator k mod A; the
+    // order depends only on b - b0
+    double acc[A];
 #pragma unroll
-      for (int i = 0; i < 8; ++i) {
-        const int b = base + i * NT;
-        x[i] = b < b1 ? part[(long)b * stride + j] : 0.0;
+    for (int k = 0; k < A; ++k) acc[k] = 0.0;
+    for (int base = b0 + threadIdx.x; base < b1; base += A * NT) {
+      double x[A];
+#pragma unroll
+      for (int k = 0; k < A; ++k) {
+        const int b = base + k * NT;
+        x[k] = b < b1 ? part[(long)b * stride + j] : 0.0;
       }
 #pragma unroll
-      for (int i = 0; i < 8; ++i) acc[i] += x[i];
+      for (int k = 0; k < A; ++k) acc[k] += x[k];
     }
-    const double v = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
+#pragma unroll
+    for (int h = A / 2; h > 0; h /= 2)
+#pragma unroll
+      for (int k = 0; k < h; ++k) acc[k] += acc[k + h];
+    const double v = acc[0];
     const double t = block_sum(v, sred);
     if (threadIdx.x == 0) out[(row * row_stride + i) * stride + j] = t * scale[i];
     __syncthreads();
