@@ -80,11 +80,10 @@ def cpu_baseline(p, ne, steps, ne1, steps1):
 
 
 def ff_bytes_per_dof(p, mixed):
-    """Face-flux array bytes per DOF: per element 3 directions x 2 sides,
-    each a chunk of 5 (p+1)^2 values padded to 16 bytes."""
+    """Face-flux array bytes per DOF: per element 3 directions x 2 sides x
+    5 states x (p+1)^2 face nodes."""
     n, w = p + 1, (4 if mixed else 8)
-    chunk = -(-5 * n * n * w // 16) * 16 // w
-    return 6 * chunk * w / (5 * n ** 3)
+    return 6 * 5 * n * n * w / (5 * n ** 3)
 
 
 def alg_bytes(dof, mixed, p=4):

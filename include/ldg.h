@@ -101,10 +101,9 @@ int ldg_rhs(ldg_handle *h, const double *u, double *k, double *Du);
    physics and precision come from cfg; there is no handle.  The operators,
    instance tables and face fluxes live in one fixed device array of
    LDG_NOALLOC_SCRATCH_BYTES per GPU (held until the process ends), so the
-   mesh must satisfy 8192 + 6 w K c <= LDG_NOALLOC_SCRATCH_BYTES with
-   w = 8 (LDG_FP64) or 4 (LDG_MIXED) bytes and c = 5 (p+1)^2 rounded up to a
-   multiple of 16 / w -- e.g. up to 22^3 elements at p = 4 in FP64 -- and a
-   larger one returns LDG_EINVAL.  Runs on `stream`
+   mesh must satisfy 8192 + 30 w K (p+1)^2 <= LDG_NOALLOC_SCRATCH_BYTES with
+   w = 8 (LDG_FP64) or 4 (LDG_MIXED) bytes -- e.g. up to 22^3 elements at
+   p = 4 in FP64 -- and a larger one returns LDG_EINVAL.  Runs on `stream`
    (a hipStream_t; NULL: the default stream) and is complete when it
    returns; calls no hipMalloc/hipFree.  All calls on one GPU share its
    array: calls that overlap in time on one GPU (several threads or

@@ -409,7 +409,7 @@ int create_impl(const ldg_config *cfg, const ldg_instance *insts, int n, ldg_han
       h->err = "device allocation failed (gradient)";
       return bad(LDG_ENOMEM);
     }
-    // face fluxes: per element 3 directions x 2 sides x (5 states x n^2 points, padded to 16 bytes)
+    // face fluxes: per element 3 directions x 2 sides x 5 states x n^2 face nodes
     const int wb = h->mixed ? (int)sizeof(float) : (int)sizeof(double);
     if (hipMalloc(&g.ff, (size_t)g.K * 6 * ff_chunk(g.n, wb) * wb) != hipSuccess) {
       h->err = "device allocation failed (face fluxes)";
@@ -1212,7 +1212,7 @@ extern "C" int ldg_rhs_noalloc(const ldg_config *cfg, const double *u, double *k
   in.group = 0;
   in.slot = 0;
   const int n = in.p + 1;
-  // the face fluxes (6 chunks of 5 n^2 values per element, each padded to 16 bytes)
+  // the face fluxes (6 chunks of 5 n^2 values per element)
   const int wb = cfg->precision == LDG_MIXED ? 4 : 8;
   const double qbytes = (double)in.K * 6 * ff_chunk(n, wb) * wb;
   if (kScrFF + qbytes > (double)LDG_NOALLOC_SCRATCH_BYTES) {

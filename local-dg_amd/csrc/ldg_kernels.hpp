@@ -3,12 +3,12 @@
 // One RK4 stage is two launches over the whole periodic mesh:
 //   ldg_grad_kernel  auxiliary gradient Q_l = (1/alpha_l)(Dn U + L0 jump_l)
 //                    (Solver::rk4Rhs first half, solver.cpp:490-508) on the
-//                    faces, and the Roe + one-sided viscous face fluxes of
-//                    both elements of every face (solver.cpp:643-698,
-//                    750-814, 891-921, 1194-1374)
-//   ldg_rhs_kernel   face lifts, the volume gradient and volume integrals,
-//                    inverse mass and the fused RK4 stage epilogue
-//                    (solver.cpp:510-535 and 410-434)
+//                    faces, the Roe + one-sided viscous face fluxes of both
+//                    elements of every face (solver.cpp:643-698, 750-814,
+//                    891-921, 1194-1374), lifted to the face nodes
+//   ldg_rhs_kernel   the volume gradient and volume integrals, minus the
+//                    lifted face terms, inverse mass and the fused RK4
+//                    stage epilogue (solver.cpp:510-535 and 410-434)
 // Both stage a block of E consecutive elements in LDS, work in 1D "lines"
 // of N = p+1 values owned by one thread (in-place sum-factorised sweeps,
 // with the even/odd split of the centro-symmetric 1D operators, symop.hpp),
@@ -29,8 +29,8 @@
 //   u0, acc, unext : FP64 (node, state, element) -- the reference layout
 //   U (stage input): the same, or in the mixed path (RK stages 2-4) FP32
 //                    fluctuations about the background in that layout
-//   FF             : T, per element [direction][side][state][face point]:
-//                    the weighted face fluxes (gradient kernel -> rhs kernel)
+//   FF             : T, per element [direction][side][state][face node]:
+//                    the lifted face fluxes (gradient kernel -> rhs kernel)
 //   Q              : TQ (node, s + 5*l, element) -- the nodal gradient, 15
 //                    fields per element, written only when asked for
 #pragma once
@@ -422,11 +422,8 @@ struct GradWaves {
 };
 
 // Face-flux array: per element, 3 directions x 2 sides, each a chunk of 5
-// states x N^2 face points padded to a whole number of 16-byte words (the
-// rhs kernel reads it in 16-byte loads).
-__host__ __device__ constexpr int ff_chunk(int n, int bytes) {
-  return (NSTATE * n * n * bytes + 15) / 16 * 16 / bytes;
-}
+// states x N^2 face nodes.
+__host__ __device__ constexpr int ff_chunk(int n, int /*bytes*/) { return NSTATE * n * n; }
 template <int N, typename T>
 __host__ __device__ constexpr int ff_per_element() { return 6 * ff_chunk(N, (int)sizeof(T)); }
 
@@ -512,7 +509,7 @@ ldg_grad_kernel(const UT *__restrict__ U, TQ *__restrict__ Q, TQ *__restrict__ F
   __shared__ C sf[E][3][NFF][NF];   // -d face of each element: nodal traces, then Gauss-point values
   __shared__ C sD[N][N + 1];
   __shared__ C sw[N];
-  __shared__ SymMat<N, C> sI;
+  __shared__ SymMat<N, C> sI, sIt;
   __shared__ long snb[E][6];
   const int lb = logical_block(la);
   if (lb < 0) return;
@@ -536,6 +533,8 @@ ldg_grad_kernel(const UT *__restrict__ U, TQ *__restrict__ Q, TQ *__restrict__ F
     constexpr int MS = sizeof(SymMat<N, C>) / sizeof(C);   // coefficients of I, one per thread
     if (tid >= 64 && tid < 64 + MS)
       reinterpret_cast<C *>(&sI)[tid - 64] = C(reinterpret_cast<const double *>(&opp->sI)[tid - 64]);
+    if (tid >= 192 && tid < 192 + MS)
+      reinterpret_cast<C *>(&sIt)[tid - 192] = C(reinterpret_cast<const double *>(&opp->sIt)[tid - 192]);
   }
   if (tid >= 128 && tid < 128 + N) sw[tid - 128] = C(opp->w[tid - 128]);
   __syncthreads();   // neighbour table
@@ -723,7 +722,8 @@ ldg_grad_kernel(const UT *__restrict__ U, TQ *__restrict__ Q, TQ *__restrict__ F
   }
   __syncthreads();
 
-  // fluxes of both sides at every face Gauss point, weighted for the lift.
+  // fluxes of both sides at every face Gauss point, weighted for the lift,
+  // into the point's slots 0-9 (read only by the thread of that point).
   // The points of one direction are padded to whole waves, so that every
   // wave evaluates one direction's flux (the direction is a template
   // parameter: a wave holding points of several directions would run each
@@ -736,7 +736,7 @@ ldg_grad_kernel(const UT *__restrict__ U, TQ *__restrict__ Q, TQ *__restrict__ F
       const unsigned dd = w / PD, r = w - dd * PD;
       if (r >= (unsigned)ne * NF) continue;
       const unsigned e = r / NF, ab = r - e * NF;
-      const C *fp = &sf[e][dd][0][ab];
+      C *fp = &sf[e][dd][0][ab];
       C outK[5], outN[5];
       if (dd == 0) face_point_flux<C, 0, NF>(fp, bg, outK, outN);
       else if (dd == 1) face_point_flux<C, 1, NF>(fp, bg, outK, outN);
@@ -744,25 +744,75 @@ ldg_grad_kernel(const UT *__restrict__ U, TQ *__restrict__ Q, TQ *__restrict__ F
       // (a select, not an index: a run-time index into m would put it in scratch)
       const double ia = dd == 0 ? m.ialpha[0] : dd == 1 ? m.ialpha[1] : m.ialpha[2];
       const C wt = sw[ab % N] * sw[ab / N] * C(ia);
-      constexpr int CH = ff_chunk(N, (int)sizeof(TQ));
-      TQ *dst = FF + k0 * ff_per_element<N, TQ>() + (e * ff_per_element<N, TQ>() + (dd * 2) * CH + ab);
 #pragma unroll
       for (int s = 0; s < NSTATE; ++s) {
-        dst[s * NF] = outN[s] * wt;                  // side 0: this element, its -d face
-        dst[CH + s * NF] = outK[s] * wt;             // side 1: the -d neighbour, its +d face
+        fp[s * NF] = outN[s] * wt;                   // side 0: this element, its -d face
+        fp[(NSTATE + s) * NF] = outK[s] * wt;        // side 1: the -d neighbour, its +d face
       }
+    }
+  }
+  __syncthreads();
+
+  // lift to the face nodes, I^T along a then along b, and write FF: per
+  // element and direction the 10 flux fields (side, state) are lines
+  // (field f, x) of N values; the b sweep stores straight to HBM
+  // (FF[element][2 d + side][state][node]), so the right-hand-side kernel
+  // only adds them
+  {
+    constexpr unsigned LL = 2 * NSTATE * N;           // lines per (element, direction)
+    constexpr int LI = (E * 3 * LL + NT - 1) / NT;
+    constexpr int CH = ff_chunk(N, (int)sizeof(TQ));
+    C *const sff = &sf[0][0][0][0];
+    unsigned lb_[LI], lf[LI], lx[LI];                 // (e 3 + d), field, x; lb_ = ~0 past the block
+#pragma unroll
+    for (int it = 0; it < LI; ++it) {
+      const unsigned w = tid + it * NT, ed = w / LL, r = w - ed * LL, f = r / N;
+      lb_[it] = w < (unsigned)ne * 3 * LL ? ed : ~0u;
+      lf[it] = f;
+      lx[it] = r - f * N;
+    }
+    SymReg<N, C> It;
+    It.load(sIt);
+#pragma unroll
+    for (int it = 0; it < LI; ++it) {
+      if (lb_[it] == ~0u) continue;
+      C *p = sff + (lb_[it] * NFF + lf[it]) * NF + N * lx[it];
+      C v[N], o[N];
+#pragma unroll
+      for (int a = 0; a < N; ++a) v[a] = p[a];
+      sym_apply<N, 1, C>(It.m, v, o);
+#pragma unroll
+      for (int a = 0; a < N; ++a) p[a] = o[a];
+    }
+    __syncthreads();
+    TQ *const ffb = FF + k0 * ff_per_element<N, TQ>();
+#pragma unroll
+    for (int it = 0; it < LI; ++it) {
+      if (lb_[it] == ~0u) continue;
+      const unsigned ed = lb_[it], f = lf[it], x = lx[it];
+      const C *p = sff + (ed * NFF + f) * NF + x;
+      C v[N], o[N];
+#pragma unroll
+      for (int b = 0; b < N; ++b) v[b] = p[N * b];
+      sym_apply<N, 1, C>(It.m, v, o);
+      // element ed / 3, chunk 2 d + side, state f mod 5, node x + N b
+      const unsigned e = ed / 3, dd = ed - 3 * e, side = f / NSTATE;
+      TQ *dst = ffb + (e * ff_per_element<N, TQ>() + (2 * dd + side) * CH + (f - NSTATE * side) * NF + x);
+#pragma unroll
+      for (int b = 0; b < N; ++b) dst[N * b] = o[b];
     }
   }
 }
 
 // ---------------------------------------------------------------------
-// Kernel B: face fluxes + volume integrals + inverse mass + RK epilogue.
+// Kernel B: volume gradient and integrals, the face terms, inverse mass,
+// RK epilogue.
 //
-// LDS per element: a 20-field volume buffer of type T with room for two
-// face field sets after it (which first holds the face fluxes of the
-// element's 6 faces, and last the FP64 inverse-mass sweeps: 5 FP64 fields
-// fit in the space of 10 FP32 fields) and the face residual (5 x N^3, type
-// T).
+// LDS per element: a 20-field volume buffer of type T (in FP32 with room
+// for the two jump face sets after it), which last holds the FP64
+// inverse-mass sweeps (5 FP64 fields fit in the space of 10 FP32 fields).
+// The face terms come from HBM into registers (the gradient kernel lifted
+// them to the face nodes).
 // ---------------------------------------------------------------------
 template <int N, typename T>
 struct RhsLds {
@@ -780,8 +830,7 @@ struct RhsLds {
   static constexpr bool JQZ = sizeof(T) == sizeof(double);
   static_assert(2 * SJ <= NSTATE * NN, "the jump sets fit the Q_z slot");
   static constexpr int SJ0 = JQZ ? 3 * NSTATE * NN : SV;   // start of the jump sets
-  static constexpr int BUF = ((JQZ ? SV : SV + 2 * SJ) + VW - 1) / VW * VW;   // per element (16-byte multiple); holds FE first
-  static_assert(FE <= SV, "face fluxes fit the volume buffer");
+  static constexpr int BUF = ((JQZ ? SV : SV + 2 * SJ) + VW - 1) / VW * VW;   // per element (16-byte multiple)
   // occupancy target (waves per SIMD) handed to the register allocator
   static constexpr int WAVES = sizeof(T) == 4 ? (N <= 5 ? 4 : 2) : (N <= 2 ? 4 : N <= 5 ? 3 : 2);
 };
@@ -841,107 +890,6 @@ struct Mat {
   }
 };
 
-// Face terms of the block: the weighted face fluxes the gradient kernel
-// wrote (FF, per element [direction][side][state][face point]) -- for each
-// element its own record's side 0 on its -d faces and the +d neighbour's
-// side 1 on its +d faces -- are staged in the volume buffer, lifted with
-// I^T along a (all directions at once) and along b, and subtracted from the
-// face nodes of the residual sr, one direction at a time (a node on an edge
-// or corner belongs to faces of several directions).
-template <int N, int E, typename T>
-__device__ inline void rhs_faces(const T *__restrict__ FF, const long (*snb)[6], long k0, int ne,
-                                 const SOps<N, T> *__restrict__ so, T (*__restrict__ sbuf)[RhsLds<N, T>::BUF],
-                                 T (*__restrict__ sr)[NSTATE][N * N * N], int tid) {
-  constexpr int NF = N * N;
-  constexpr int CH = RhsLds<N, T>::CH, FE = RhsLds<N, T>::FE;
-  using V = typename Vec16<T>::type;
-  constexpr int VC = CH * (int)sizeof(T) / 16;      // 16-byte words per chunk
-  constexpr int FL = (E * 6 * VC + NT - 1) / NT;
-  // 16-byte loads of the 6 chunks (direction, side) of every element, all
-  // issued first: side 0 from the element's own record, side 1 from the +d
-  // neighbour's
-  {
-    V v[FL];
-#pragma unroll
-    for (int it = 0; it < FL; ++it) {
-      const unsigned w = tid + it * NT;
-      const unsigned ec = w / VC, j = w - ec * VC;  // ec = e * 6 + chunk
-      const unsigned e = ec / 6, c = ec - 6 * e;
-      const unsigned es = e < (unsigned)ne ? e : 0;
-      const long K = (c & 1) ? snb[es][c | 1] : k0 + es;   // c | 1 = 2 dd + 1: the +d neighbour
-      v[it] = reinterpret_cast<const V *>(FF + K * FE + c * CH)[j];
-    }
-#pragma unroll
-    for (int it = 0; it < FL; ++it) {
-      const unsigned w = tid + it * NT;
-      const unsigned ec = w / VC, j = w - ec * VC;
-      const unsigned e = ec / 6;
-      if (e < (unsigned)ne) reinterpret_cast<V *>(&sbuf[e][(ec - 6 * e) * CH])[j] = v[it];
-    }
-  }
-  __syncthreads();
-  // I^T along a, in place: lines of N values at (chunk, state, qb)
-  {
-    constexpr int LC = NSTATE * N;                  // lines per chunk
-    SymReg<N, T> It;
-    It.load(so->It);
-    for (unsigned w = tid; w < (unsigned)ne * 6 * LC; w += NT) {
-      const unsigned ec = w / LC, r = w - ec * LC;
-      const unsigned e = ec / 6, c = ec - 6 * e;
-      T *p = &sbuf[e][c * CH + N * r];
-      T v[N], o[N];
-#pragma unroll
-      for (int q = 0; q < N; ++q) v[q] = p[q];
-      sym_apply<N, 1, T>(It.m, v, o);
-#pragma unroll
-      for (int i = 0; i < N; ++i) p[i] = o[i];
-    }
-  }
-  __syncthreads();
-  // I^T along b, subtracted into the face nodes of the residual, one
-  // direction at a time; a line is (e, side, s, a), the same for the three
-  // directions, so a thread decomposes its lines once
-  SymReg<N, T> It;
-  It.load(so->It);
-  constexpr int BL = 2 * NSTATE * N;              // lines per element and direction
-  constexpr int BI = (E * BL + NT - 1) / NT;
-  unsigned bsrc[BI], bdst[BI];                    // line start in sbuf[0] (chunk 0) and in sr[0], ~0 past the block
-  bool bside[BI];
-#pragma unroll
-  for (int it = 0; it < BI; ++it) {
-    const unsigned w = tid + it * NT;
-    const unsigned e = w / BL, r = w - e * BL;
-    const unsigned side = r / (NSTATE * N), sa = r - side * (NSTATE * N);
-    const unsigned s = sa / N, a = sa - s * N;
-    bside[it] = side != 0;
-    bsrc[it] = w < (unsigned)ne * BL ? e * RhsLds<N, T>::BUF + side * CH + s * NF + a : ~0u;
-    bdst[it] = (e * NSTATE + s) * (N * N * N);
-    // the face coordinate a of direction d has memory stride N (x faces) or 1
-    bdst[it] |= a << 24;
-  }
-#pragma unroll
-  for (int dd = 0; dd < 3; ++dd) {
-    constexpr int ST[3] = {1, N, N * N};
-    const int st_d = ST[dd];
-    const int ast = dd == 0 ? N : 1;          // memory stride of the face coordinate a
-    const int bst = dd == 2 ? N : N * N;      // memory stride of the face coordinate b
-#pragma unroll
-    for (int it = 0; it < BI; ++it) {
-      if (bsrc[it] == ~0u) continue;
-      const T *p = &sbuf[0][bsrc[it] + dd * 2 * CH];
-      T v[N], o[N];
-#pragma unroll
-      for (int q = 0; q < N; ++q) v[q] = p[q * N];
-      sym_apply<N, 1, T>(It.m, v, o);
-      const unsigned a = bdst[it] >> 24;
-      T *dst = &sr[0][0][(bdst[it] & 0xffffffu) + a * ast + (bside[it] ? (N - 1) * st_d : 0)];
-#pragma unroll
-      for (int b = 0; b < N; ++b) dst[b * bst] -= o[b];
-    }
-    __syncthreads();
-  }
-}
-
 template <int N, int E, typename T, typename UT>
 __global__ void __launch_bounds__(NT, (RhsLds<N, T>::WAVES))
 ldg_rhs_kernel(const UT *__restrict__ U, const T *__restrict__ FF, LaunchArgs la, const Ops<N, double> *__restrict__ opp,
@@ -949,7 +897,6 @@ ldg_rhs_kernel(const UT *__restrict__ U, const T *__restrict__ FF, LaunchArgs la
   using L = RhsLds<N, T>;
   constexpr int NN = L::NN, NF = L::NF, BUF = L::BUF;
   __shared__ __align__(16) T sbuf[E][BUF];
-  __shared__ T sr[E][NSTATE][NN];
   __shared__ double sred[NT / 64];
   __shared__ double sbg[NSTATE];      // the background state (rho0, 0, 0, 0, E0)
   __shared__ SOps<N, T> sops;
@@ -995,6 +942,10 @@ ldg_rhs_kernel(const UT *__restrict__ U, const T *__restrict__ FF, LaunchArgs la
   constexpr int XI = XSPLIT ? 1 : (int)((KP + NT - 1) / NT);
   const unsigned totx = ne * XL;
   UT vo[XI][N], vx[XI], vy[XI][N], vz[XI][N];
+  // the item's face terms (lifted to the nodes by the gradient kernel): the
+  // -x face at the line's first node, the +x face at its last, and the
+  // line's nodes on a -y/+y and a -z/+z face where it lies on one
+  T fa[XI], fb[XI], fy[XI][N], fz[XI][N];
   // Line items (e, s, ab) of the sweep phases below run in kinds of KP
   // items (a kind = one family of lines, e.g. the z-lines of F_x).  When KP
   // divides the workgroup (p = 4: E = 2 fills it, E = 1 half of it) a
@@ -1051,7 +1002,6 @@ ldg_rhs_kernel(const UT *__restrict__ U, const T *__restrict__ FF, LaunchArgs la
 
   fill_neighbors(m, ec, d.elem_off, ne, snb, tid);
   load_sops<N, T>(&sops, opp, m.ialpha[0], m.ialpha[1], m.ialpha[2], tid);
-  for (int w = tid; w < ne * NSTATE * NN; w += NT) (&sr[0][0][0])[w] = T(0);
   if (tid < NSTATE) sbg[tid] = tid == 0 ? ph.rho0 : tid == NSTATE - 1 ? ph.E0 : 0.0;
   __syncthreads();
   // the neighbours' traces the jumps need: the -x neighbour's node at the
@@ -1072,10 +1022,20 @@ ldg_rhs_kernel(const UT *__restrict__ U, const T *__restrict__ FF, LaunchArgs la
       vy[it][q] = ny[q];
       vz[it][q] = nz[q];
     }
+    // FF[element][2 d + side][state][node]: side 0 of the element's own
+    // record for its -d faces, side 1 of the +d neighbour's for its +d faces
+    constexpr int CH = L::CH, FE = L::FE;
+    const T *const ffo = FF + (k0 + e) * FE + s * NF;
+    fa[it] = ffo[ab];
+    fb[it] = FF[snb[e][1] * FE + (CH + s * NF + ab)];
+    const T *fyp = iy == N - 1 ? FF + snb[e][3] * FE + (3 * CH + s * NF + N * iz) : ffo + 2 * CH + N * iz;
+    const T *fzp = iz == N - 1 ? FF + snb[e][5] * FE + (5 * CH + s * NF + N * iy) : ffo + 4 * CH + N * iy;
+#pragma unroll
+    for (int q = 0; q < N; ++q) {
+      fy[it][q] = fyp[q];
+      fz[it][q] = fzp[q];
+    }
   }
-
-  // ================= faces: lift the gradient kernel's fluxes =================
-  rhs_faces<N, E, T>(FF, snb, k0, ne, so, sbuf, sr, tid);
 
   // ================= volume =================
   // The state and its gradient at the volume Gauss points, from the state
@@ -1317,7 +1277,9 @@ ldg_rhs_kernel(const UT *__restrict__ U, const T *__restrict__ FF, LaunchArgs la
     bool has[XR];
     SymReg<N, T> It;
     It.load(so->It);
-    auto xline = [&](const LineIt &L, double (&out)[N]) {
+    // volume minus the face terms of the line's nodes (the gradient
+    // kernel's lifted face fluxes, loaded with the x-stage lines: item it)
+    auto xline = [&](const LineIt &L, int it, double (&out)[N]) {
       const unsigned base = N * L.ab;
       const T *px = &sbuf[L.e][(NSTATE + L.s) * NN + base];
       const T *pg = px + NSTATE * NN;
@@ -1326,19 +1288,30 @@ ldg_rhs_kernel(const UT *__restrict__ U, const T *__restrict__ FF, LaunchArgs la
 #pragma unroll
       for (int q = 0; q < N; ++q) { vx[q] = px[q]; vg[q] = pg[q]; }
       sym_apply2<N, -1, 1, T>(so->Dt, vx, It.m, vg, o);
+      o[0] -= fa[it];
+      o[N - 1] -= fb[it];
+      const unsigned iy = L.ab % N, iz = L.ab / N;
+      if (iy == 0 || iy == N - 1) {
 #pragma unroll
-      for (int i = 0; i < N; ++i) rr[i] = (double)(sr[L.e][L.s][base + i] + o[i]);
+        for (int i = 0; i < N; ++i) o[i] -= fy[it][i];
+      }
+      if (iz == 0 || iz == N - 1) {
+#pragma unroll
+        for (int i = 0; i < N; ++i) o[i] -= fz[it][i];
+      }
+#pragma unroll
+      for (int i = 0; i < N; ++i) rr[i] = (double)o[i];
       sym_apply<N, 1, double>(so->Mi, rr, out);
     };
     if constexpr (HOIST) {
       has[0] = tid < (int)KP && own_i < totx;
-      if (has[0]) xline(own, res[0]);
+      if (has[0]) xline(own, 0, res[0]);
     } else {
 #pragma unroll
       for (int it = 0; it < XR; ++it) {
         const unsigned w = tid + it * NT;
         has[it] = w < totx;
-        if (has[it]) xline(line_at(w), res[it]);
+        if (has[it]) xline(line_at(w), it, res[it]);
       }
     }
     __syncthreads();

@@ -11,9 +11,8 @@ import bench  # noqa: E402
 
 
 def test_alg_bytes_per_dof():
-    # p = 4: an element has 625 DOF and 6 face-flux chunks of 5 x 25 values,
-    # padded to 16 bytes: 126 doubles or 128 floats.
-    ff64, ff32 = 6 * 126 * 8 / 625, 6 * 128 * 4 / 625
+    # p = 4: an element has 625 DOF and 6 face-flux chunks of 5 x 25 values
+    ff64, ff32 = 6 * 125 * 8 / 625, 6 * 125 * 4 / 625
     assert abs(bench.ff_bytes_per_dof(4, False) - ff64) < 1e-12 and abs(bench.ff_bytes_per_dof(4, True) - ff32) < 1e-12
     # FP64: grad U 8 + FF; rhs stages (u^n once, acc write, U_2), 2 x (U, u^n,
     # acc r+w, U_next), (U, u^n, acc r+w), each + FF

@@ -75,7 +75,7 @@ def test_noalloc_refuses_meshes_beyond_its_scratch():
     hip = _hip()
     g = ldg_amd.LDG(4, 3)
     cfg = g.cfg
-    cfg.nx = cfg.ny = cfg.nz = 23            # 8192 + 23^3 x 6 x 126 x 8 bytes > 64 MiB
+    cfg.nx = cfg.ny = cfg.nz = 23            # 8192 + 23^3 x 30 x 25 x 8 bytes > 64 MiB
     nbytes = 23 ** 3 * 5 * 125 * 8            # full-size device arrays, although the call is refused first
     du, dk = ctypes.c_void_p(), ctypes.c_void_p()
     assert hip.hipMalloc(ctypes.byref(du), nbytes) == 0 and hip.hipMalloc(ctypes.byref(dk), nbytes) == 0
