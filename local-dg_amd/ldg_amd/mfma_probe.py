@@ -1,0 +1,154 @@
+"""Times the rhs kernel's y/z test-function contraction at p = 4 in three
+forms (libldg_probe.so, csrc/mfma_probe.hip): the solver's vector sweeps,
+FP32-input MFMA and FP16 hi/lo MFMA on the dense 25x25 slab operators, on
+64^3 elements of random flux fields, and checks each against an FP64
+evaluation of the same contraction.
+
+    python -m ldg_amd.mfma_probe [--elements K] [--reps R]
+
+Prints one JSON line.  Each launch stages 4 elements per workgroup in LDS
+and repeats the contraction R times there, in place (every repetition reads
+what the one before wrote, as the solver's sweeps overwrite their inputs);
+the time of one contraction is (t(R) - t(1)) / (R - 1), which leaves out the
+HBM transfer of the fields."""
+import argparse
+import ctypes
+import json
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+N = 5
+
+
+def basis(p=4):
+    """1D interpolation I[q][j] = l_j(g_q) and derivative D[q][j] = l_j'(g_q)
+    of the Lagrange basis on the Chebyshev-Gauss-Lobatto nodes at the Gauss
+    points."""
+    n = p + 1
+    r = -np.cos(np.pi * np.arange(n) / p)
+    g, _ = np.polynomial.legendre.leggauss(n)
+    I = np.zeros((n, n))
+    D = np.zeros((n, n))
+    for j in range(n):
+        others = [m for m in range(n) if m != j]
+        den = np.prod([r[j] - r[m] for m in others])
+        for q in range(n):
+            I[q, j] = np.prod([g[q] - r[m] for m in others]) / den
+            D[q, j] = sum(np.prod([g[q] - r[m] for m in others if m != k]) for k in others) / den
+    return I, D
+
+
+def symmetrize(A, S):
+    return 0.5 * (A + S * A[::-1, ::-1])
+
+
+def symmat(A):
+    """SymMat<5, float>: P[3][2], R[3][2], C[3], A[1][1] (the even/odd split form)."""
+    P = np.array([[(A[q, j] + A[q, N - 1 - j]) / 2 for j in range(2)] for q in range(3)])
+    R = np.array([[(A[q, j] - A[q, N - 1 - j]) / 2 for j in range(2)] for q in range(3)])
+    C = np.array([A[q, 2] for q in range(3)])
+    return np.concatenate([P.ravel(), R.ravel(), C, [0.0]]).astype(np.float32)
+
+
+def operators():
+    I, D = basis()
+    It = symmetrize(I.T.copy(), 1)
+    Dt = symmetrize(D.T.copy(), -1)
+    # slab index i = y + 5 z: kron(Z op, Y op)
+    Ax = np.kron(It, It)
+    Ag = np.hstack([np.kron(It, Dt), np.kron(Dt, It)])
+    return It, Dt, Ax, Ag
+
+
+def pack(It, Dt, Ax, Ag):
+    ax = np.zeros((32, 26), np.float32)
+    ax[:25, :25] = Ax
+    ag = np.zeros((32, 50), np.float32)
+    ag[:25, :] = Ag
+
+    def hilo(A, k):
+        a = np.zeros((32, k))
+        a[:A.shape[0], :A.shape[1]] = A
+        h = a.astype(np.float16)
+        lo = (a - h.astype(np.float64)).astype(np.float16)
+        return h, lo
+
+    axh, axl = hilo(Ax, 32)
+    agh, agl = hilo(Ag, 64)
+    return b"".join([symmat(It).tobytes(), symmat(Dt).tobytes(), ax.tobytes(), ag.tobytes(),
+                     axh.tobytes(), axl.tobytes(), agh.tobytes(), agl.tobytes()])
+
+
+def reference(F, Ax, Ag, elems):
+    """FP64 O_x, O_g of the sampled elements: F (K, 15, 125) -> (len, 10, 125)."""
+    out = np.zeros((len(elems), 10, 125))
+    for t, e in enumerate(elems):
+        f = F[e].astype(np.float64).reshape(15, 25, 5)      # [field][i = y + 5 z][x]
+        out[t, :5] = np.einsum("ab,sbx->sax", Ax, f[0:5]).reshape(5, 125)
+        out[t, 5:] = (np.einsum("ab,sbx->sax", Ag[:, :25], f[5:10])
+                      + np.einsum("ab,sbx->sax", Ag[:, 25:], f[10:15])).reshape(5, 125)
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--elements", type=int, default=64 ** 3)
+    ap.add_argument("--reps", type=int, default=16)
+    args = ap.parse_args()
+    import torch
+    lib = ctypes.CDLL(os.path.join(HERE, "libldg_probe.so"))
+    lib.ldg_probe_contract.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_long,
+                                       ctypes.c_void_p, ctypes.c_void_p]
+    It, Dt, Ax, Ag = operators()
+    blob = pack(It, Dt, Ax, Ag)
+    assert len(blob) == lib.ldg_probe_ops_bytes(), (len(blob), lib.ldg_probe_ops_bytes())
+    K = args.elements
+    rng = np.random.default_rng(0)
+    F = rng.standard_normal((K, 15, 125)).astype(np.float32)
+    dF = torch.from_numpy(F).cuda()
+    dO = torch.empty((K, 10, 125), dtype=torch.float32, device="cuda")
+    dops = torch.from_numpy(np.frombuffer(blob, dtype=np.uint8).copy()).cuda()
+    stream = torch.cuda.current_stream()
+    elems = rng.choice(K, size=64, replace=False)
+    ref = reference(F, Ax, Ag, elems)
+
+    def launch(v, reps):
+        rc = lib.ldg_probe_contract(v, reps, dF.data_ptr(), dO.data_ptr(), K, dops.data_ptr(),
+                                    ctypes.c_void_p(stream.cuda_stream))
+        assert rc == 0, rc
+
+    def timed(v, reps, n=5):
+        launch(v, reps)
+        best = None
+        for _ in range(n):
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record(stream)
+            launch(v, reps)
+            b.record(stream)
+            b.synchronize()
+            t = a.elapsed_time(b)
+            best = t if best is None else min(best, t)
+        return best
+
+    out = {"probe": "y/z test-function contraction of 15 flux fields, p = 4", "elements": K,
+           "reps": args.reps, "variants": {}}
+    names = {0: "valu_sum_factorised", 1: "mfma_f32_32x32x2_slab", 2: "mfma_f16_hilo_32x32x16_slab"}
+    for v in (0, 1, 2):
+        t1 = timed(v, 1)
+        launch(v, 1)
+        torch.cuda.synchronize()
+        o = dO.cpu().numpy()[elems].astype(np.float64)
+        err = float(np.linalg.norm(o - ref) / np.linalg.norm(ref))
+        tr = timed(v, args.reps)
+        per = (tr - t1) / (args.reps - 1)
+        out["variants"][names[v]] = {"ms_one_pass_with_hbm": t1, "ms_reps": tr, "ms_per_contraction": per,
+                                     "ns_per_element": per * 1e6 / K, "rel_l2_vs_fp64": err}
+        print(f"{names[v]}: {t1:.4f} ms (1 rep), {tr:.4f} ms ({args.reps} reps), {per:.4f} ms per contraction, "
+              f"error {err:.2e}", flush=True)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()
