@@ -92,40 +92,40 @@ def reference(F, Ax, Ag, elems):
     return out
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--elements", type=int, default=64 ** 3)
-    ap.add_argument("--reps", type=int, default=16)
-    args = ap.parse_args()
+def measure(elements=64 ** 3, reps=16, samples=64, log=None):
+    """Times the three variants on `elements` random elements (a multiple of
+    4) and checks each against the FP64 contraction on `samples` of them."""
     import torch
     lib = ctypes.CDLL(os.path.join(HERE, "libldg_probe.so"))
     lib.ldg_probe_contract.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_long,
                                        ctypes.c_void_p, ctypes.c_void_p]
     It, Dt, Ax, Ag = operators()
     blob = pack(It, Dt, Ax, Ag)
-    assert len(blob) == lib.ldg_probe_ops_bytes(), (len(blob), lib.ldg_probe_ops_bytes())
-    K = args.elements
+    if len(blob) != lib.ldg_probe_ops_bytes():
+        raise RuntimeError(f"operator block {len(blob)} bytes, the library expects {lib.ldg_probe_ops_bytes()}")
+    K = elements
     rng = np.random.default_rng(0)
     F = rng.standard_normal((K, 15, 125)).astype(np.float32)
     dF = torch.from_numpy(F).cuda()
     dO = torch.empty((K, 10, 125), dtype=torch.float32, device="cuda")
     dops = torch.from_numpy(np.frombuffer(blob, dtype=np.uint8).copy()).cuda()
     stream = torch.cuda.current_stream()
-    elems = rng.choice(K, size=64, replace=False)
+    elems = rng.choice(K, size=min(samples, K), replace=False)
     ref = reference(F, Ax, Ag, elems)
 
-    def launch(v, reps):
-        rc = lib.ldg_probe_contract(v, reps, dF.data_ptr(), dO.data_ptr(), K, dops.data_ptr(),
+    def launch(v, r):
+        rc = lib.ldg_probe_contract(v, r, dF.data_ptr(), dO.data_ptr(), K, dops.data_ptr(),
                                     ctypes.c_void_p(stream.cuda_stream))
-        assert rc == 0, rc
+        if rc != 0:
+            raise RuntimeError(f"ldg_probe_contract returned {rc}")
 
-    def timed(v, reps, n=5):
-        launch(v, reps)
+    def timed(v, r, n=5):
+        launch(v, r)
         best = None
         for _ in range(n):
             a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             a.record(stream)
-            launch(v, reps)
+            launch(v, r)
             b.record(stream)
             b.synchronize()
             t = a.elapsed_time(b)
@@ -133,7 +133,7 @@ def main():
         return best
 
     out = {"probe": "y/z test-function contraction of 15 flux fields, p = 4", "elements": K,
-           "reps": args.reps, "variants": {}}
+           "reps": reps, "variants": {}}
     names = {0: "valu_sum_factorised", 1: "mfma_f32_32x32x2_slab", 2: "mfma_f16_hilo_32x32x16_slab"}
     for v in (0, 1, 2):
         t1 = timed(v, 1)
@@ -141,12 +141,22 @@ def main():
         torch.cuda.synchronize()
         o = dO.cpu().numpy()[elems].astype(np.float64)
         err = float(np.linalg.norm(o - ref) / np.linalg.norm(ref))
-        tr = timed(v, args.reps)
-        per = (tr - t1) / (args.reps - 1)
+        tr = timed(v, reps)
+        per = (tr - t1) / (reps - 1)
         out["variants"][names[v]] = {"ms_one_pass_with_hbm": t1, "ms_reps": tr, "ms_per_contraction": per,
                                      "ns_per_element": per * 1e6 / K, "rel_l2_vs_fp64": err}
-        print(f"{names[v]}: {t1:.4f} ms (1 rep), {tr:.4f} ms ({args.reps} reps), {per:.4f} ms per contraction, "
-              f"error {err:.2e}", flush=True)
+        if log:
+            log(f"{names[v]}: {t1:.4f} ms (1 rep), {tr:.4f} ms ({reps} reps), {per:.4f} ms per contraction, "
+                f"error {err:.2e}")
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--elements", type=int, default=64 ** 3)
+    ap.add_argument("--reps", type=int, default=16)
+    args = ap.parse_args()
+    out = measure(args.elements, args.reps, log=lambda m: print(m, flush=True))
     print(json.dumps(out), flush=True)
 
 
