@@ -244,14 +244,18 @@ def generate(closure, runs=4, p_fine=5, ne=24, t_end=10.0, dt_snap=0.1, keep=0.2
 
 
 def run_tgv(p, ne, t_end, closure=None, bound=5.0, precision=LDG_FP64, cfl=0.01, dt_diag=0.1, Re=1600.0,
-            M0=0.1, log=None, dissipative=False, ic_from=None):
+            M0=0.1, log=None, dissipative=False, ic_from=None, t_on=0.0):
     """Taylor-Green at order p on ne^3 to t_end (reference time-step rule);
     returns times and reference KE after every step, Gauss-point KE and
     enstrophy every dt_diag, wall time of the stepping, gated elements.
     ic_from = q > p starts from the L2 projection of the order-q initial
-    state (the states the closure is trained on) instead of the nodal one."""
+    state (the states the closure is trained on) instead of the nodal one.
+    The closure is attached from time t_on on (rounded to a multiple of
+    dt_diag); before that the run is the plain order-p run."""
     g = LDG(p, ne, Re=Re, M0=M0, precision=precision)
-    if closure is not None:
+    k_on = int(round(t_on / dt_diag))
+    attached = closure is not None and k_on <= 0
+    if attached:
         attach(g, closure, bound, dissipative)
     if ic_from:
         f = LDG(ic_from, ne, Re=Re, M0=M0)
@@ -267,6 +271,9 @@ def run_tgv(p, ne, t_end, closure=None, bound=5.0, precision=LDG_FP64, cfl=0.01,
     off_max = 0
     diverged = None
     for k in range(ndiag):
+        if closure is not None and not attached and k >= k_on:
+            attach(g, closure, bound, dissipative)
+            attached = True
         t0 = time.perf_counter()
         try:
             kes.append(g.step(dt, n_per, record_ke=True))
@@ -275,7 +282,7 @@ def run_tgv(p, ne, t_end, closure=None, bound=5.0, precision=LDG_FP64, cfl=0.01,
             break
         wall += time.perf_counter() - t0
         diag.append(g.diagnostics())
-        if closure is not None:
+        if attached:
             off_max = max(off_max, gated(g)[0])
         if log and k % 10 == 9:
             log(f"p={p} t={(k + 1) * dt_diag:.1f} KE={diag[-1]['ke']:.6f} wall={wall:.1f}s")
@@ -302,6 +309,34 @@ def compare(run, ref):
     d_ref = dissipation_curve(ref["t_diag"], ref["ke_gauss"])
     d_run = np.interp(ref["t_diag"], run["t_diag"], d_run)
     return float(np.max(np.abs(d_run - d_ref)) / np.max(np.abs(d_ref)))
+
+
+def projected_run(p_fine, p, ne, t_end, Re=1600.0, M0=0.1, cfl=0.01, dt_diag=0.1, log=None):
+    """The order-p_fine Taylor-Green run and its L2 projection onto order p
+    every dt_diag: the Gauss-point KE of both.  A closure trained on the
+    target P rhs_fine(u) - rhs_p(P u) makes the corrected order-p run follow
+    the projection P u (exactly, if the closure were exact and the run
+    started from P u(0)), so -d/dt of the projection's KE is the
+    dissipation curve such a closure can reach at best."""
+    f = LDG(p_fine, ne, Re=Re, M0=M0)
+    g = LDG(p, ne, Re=Re, M0=M0)
+    f.initial_condition()
+    n_per, dt = steps_to(f.time_step(cfl), dt_diag)
+    ndiag = int(round(t_end / dt_diag))
+    ke_f, ke_p = [], []
+    for k in range(ndiag + 1):
+        project_state(f, 0, g, 0)
+        ke_f.append(f.diagnostics()["ke"])
+        ke_p.append(g.diagnostics()["ke"])
+        if k == ndiag:
+            break
+        f.step(dt, n_per)
+        if log and k % 10 == 9:
+            log(f"p={p_fine} t={(k + 1) * dt_diag:.1f} KE={ke_f[-1]:.6f} projected KE={ke_p[-1]:.6f}")
+    f.close()
+    g.close()
+    t = np.arange(ndiag + 1) * dt_diag
+    return {"t_diag": t, "ke_gauss": np.array(ke_f)}, {"t_diag": t, "ke_gauss": np.array(ke_p)}
 
 
 def apriori(closure, times, ne=24, p_fine=5, Re=1600.0, M0=0.1, seed=12345, precision=LDG_MIXED, cfl=0.01,
@@ -364,16 +399,34 @@ def main(argv=None):
     ev.add_argument("--gates", default="plain", help="comma-separated: plain, dissipative")
     ev.add_argument("--relax", default="1", help="comma-separated factors multiplying the correction")
     ev.add_argument("--ic-from", type=int, default=0, help="start p=2 runs from the projected order-q state")
+    ev.add_argument("--t-on", default="0", help="comma-separated times from which the closure is attached")
     ev.add_argument("--ref-p", type=int, default=5)
     ev.add_argument("--ref-json", default=None, help="take the resolved run from an earlier evaluate output")
     ev.add_argument("--no-ref", action="store_true")
     ev.add_argument("--out", default=None)
+    pr = sub.add_parser("projected")
+    pr.add_argument("--p-fine", type=int, default=5)
+    pr.add_argument("--ne", type=int, default=24)
+    pr.add_argument("--t-end", type=float, default=10.0)
+    pr.add_argument("--out", default=None)
     ap_ = sub.add_parser("apriori")
     ap_.add_argument("--weights", required=True)
     ap_.add_argument("--ne", type=int, default=24)
     ap_.add_argument("--times", default="0,1,2,4,6,8,9.5")
     a = ap.parse_args(argv)
     log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
+    if a.cmd == "projected":
+        fine, proj = projected_run(a.p_fine, 2, a.ne, a.t_end, log=log)
+        d_f = dissipation_curve(fine["t_diag"], fine["ke_gauss"])
+        d_p = dissipation_curve(proj["t_diag"], proj["ke_gauss"])
+        res = {"dEk_dt_err_projected": compare(proj, fine), "t": fine["t_diag"].tolist(),
+               "ke_fine": fine["ke_gauss"].tolist(), "ke_projected": proj["ke_gauss"].tolist(),
+               "dissipation_fine": d_f.tolist(), "dissipation_projected": d_p.tolist()}
+        if a.out:
+            with open(a.out, "w") as fo:
+                json.dump(res, fo)
+        print(json.dumps({"dEk_dt_err_projected": res["dEk_dt_err_projected"]}), flush=True)
+        return
     if a.cmd == "apriori":
         c = Closure.load(a.weights)
         res = apriori(c, [float(t) for t in a.times.split(",")], a.ne, log=log)
@@ -405,8 +458,10 @@ def main(argv=None):
             c.set(norm=n2)
             for bound in [float(b) for b in a.bounds.split(",")]:
                 for gate in a.gates.split(","):
-                    runs[f"closure_b{bound:g}_{gate}_r{relax:g}"] = run_tgv(
-                        2, a.ne, a.t_end, c, bound, log=log, dissipative=(gate == "dissipative"), ic_from=a.ic_from)
+                    for t_on in [float(t) for t in a.t_on.split(",")]:
+                        tag = f"closure_b{bound:g}_{gate}_r{relax:g}" + (f"_on{t_on:g}" if t_on > 0 else "")
+                        runs[tag] = run_tgv(2, a.ne, a.t_end, c, bound, log=log, dissipative=(gate == "dissipative"),
+                                            ic_from=a.ic_from, t_on=t_on)
         c.set(norm=norm)
         runs["plain_p2"] = run_tgv(2, a.ne, a.t_end, log=log, ic_from=a.ic_from)
         ref = None

@@ -280,6 +280,19 @@ def test_corrected_run_is_deterministic():
     np.testing.assert_array_equal(out[0][1], out[1][1])
 
 
+def test_late_switch_on_matches_plain_until_attached():
+    """run_tgv(t_on=...): the run is the plain run bit for bit until t_on,
+    and differs after the closure is attached there."""
+    c = C.Closure(2, 256, 0, seed=21)
+    c.set(norm=random_norm(c, np.random.default_rng(7)))
+    plain = C.run_tgv(2, 4, 0.3, dt_diag=0.1)
+    late = C.run_tgv(2, 4, 0.3, closure=c, bound=1e30, dt_diag=0.1, t_on=0.2)
+    n_on = 2 * plain["steps"] // 3                  # steps before t = 0.2
+    np.testing.assert_array_equal(late["ke_ref"][:n_on], plain["ke_ref"][:n_on])
+    np.testing.assert_array_equal(late["ke_gauss"][:3], plain["ke_gauss"][:3])
+    assert late["ke_gauss"][3] != plain["ke_gauss"][3]
+
+
 def test_adam_matches_numpy_adam():
     """Five full-batch Adam steps of ldg_closure_train against Algorithm 1
     of Kingma & Ba (2015) in float64 numpy: m_t = b1 m + (1-b1) g,
