@@ -1356,31 +1356,50 @@ ldg_rhs_kernel(const UT *__restrict__ U, const T *__restrict__ FF, LaunchArgs la
       }
     }
   }
-  // Epilogue items: the block's values are contiguous in every (node,
-  // state, element) array, so item w (< ne 5 N^3) is word gofs + w there
-  // (uniform base, 32-bit offset) and word li = e BUFD + r of the FP64
-  // view of the volume buffer (element e, r = w - 5 N^3 e).
+  // Epilogue items.  The block's values are contiguous in every (node,
+  // state, element) array: group g = 5 e + s (element e, state s) is words
+  // g N^3 .. g N^3 + N^3 - 1 past the block's base gofs (uniform base,
+  // 32-bit offset), and words e BUFD + s N^3 + q of the FP64 view of the
+  // volume buffer.  Item (it, thread t) is word w = t + it NT.  When N^3 is
+  // a multiple of the wave (N = 4) a wave's items lie in one group, which
+  // is then computed as a wave-uniform value (no per-lane division).
   constexpr int EPI = (E * NSTATE * NN + NT - 1) / NT;
+  constexpr bool GS = NN % 64 == 0;
   constexpr unsigned BUFD = L::BUF * sizeof(T) / sizeof(double);   // an element's buffer, in doubles
   const unsigned tot_epi = ne * NSTATE * NN;
   const long gofs = k0 * NSTATE * NN;
-  unsigned li[EPI];
-#pragma unroll
-  for (int it = 0; it < EPI; ++it) {
-    const unsigned w = tid + it * NT, e = w / (NSTATE * NN);
-    li[it] = w + e * (BUFD - NSTATE * NN);
-  }
+  struct EpiItem {
+    bool ok;
+    unsigned w, li, s;   // word past gofs, word of the FP64 view, state
+  };
+  auto epi_item = [&](int it) -> EpiItem {
+    EpiItem x;
+    unsigned g, q;
+    if constexpr (GS) {
+      g = (unsigned)__builtin_amdgcn_readfirstlane((int)((tid + it * NT) / NN));
+      q = (unsigned)tid % NN;
+    } else {
+      g = (tid + it * NT) / NN;
+      q = tid + it * NT - g * NN;
+    }
+    const unsigned e = g / NSTATE;
+    x.w = g * NN + q;
+    x.ok = x.w < tot_epi;
+    x.s = g - e * NSTATE;
+    x.li = x.w + e * (BUFD - NSTATE * NN);
+    return x;
+  };
   double ru[EPI], ra[EPI];
   {
     const double *const u0b = st.u0 + gofs, *const accb = st.acc + gofs;
 #pragma unroll
     for (int it = 0; it < EPI; ++it) {
-      const unsigned w = tid + it * NT;
+      const EpiItem x = epi_item(it);
       ru[it] = 0.0;
       ra[it] = 0.0;
-      if (!PF && w < tot_epi) {
-        if (need_u0) ru[it] = u0b[w];
-        if (need_acc) ra[it] = accb[w];
+      if (!PF && x.ok) {
+        if (need_u0) ru[it] = u0b[x.w];
+        if (need_acc) ra[it] = accb[x.w];
       }
     }
   }
@@ -1415,18 +1434,17 @@ ldg_rhs_kernel(const UT *__restrict__ U, const T *__restrict__ FF, LaunchArgs la
   // ---- epilogue (FP64): k, RK4 accumulation and next stage input.  All of
   // a thread's HBM reads are issued before any is used.
   {
-    const unsigned tot = tot_epi;
     double *const mb = md(0);
     double kv[EPI];
 #pragma unroll
     for (int it = 0; it < EPI; ++it) {
-      const unsigned w = tid + it * NT;
+      const EpiItem x = epi_item(it);
       kv[it] = 0.0;
-      if (w < tot) {
-        if (PF && need_u0) ru[it] = mb[li[it] + NSTATE * NN];
-        if (PF && need_acc) ra[it] = mb[li[it] + 2 * NSTATE * NN];
-        kv[it] = mb[li[it]];
-        if (st.corr && st.gate[k0 + w / (NSTATE * NN)]) kv[it] += st.corr[gofs + w];
+      if (x.ok) {
+        if (PF && need_u0) ru[it] = mb[x.li + NSTATE * NN];
+        if (PF && need_acc) ra[it] = mb[x.li + 2 * NSTATE * NN];
+        kv[it] = mb[x.li];
+        if (st.corr && st.gate[k0 + x.w / (NSTATE * NN)]) kv[it] += st.corr[gofs + x.w];
       }
     }
     if (st.ke_part) __syncthreads();   // every k read before the new state reuses the buffer
@@ -1434,16 +1452,17 @@ ldg_rhs_kernel(const UT *__restrict__ U, const T *__restrict__ FF, LaunchArgs la
     float *const next32b = st.next32 + gofs, *const unext32b = st.unext32 + gofs;
 #pragma unroll
     for (int it = 0; it < EPI; ++it) {
-      const unsigned w = tid + it * NT;
-      if (w < tot) {
+      const EpiItem x = epi_item(it);
+      if (x.ok) {
+        const unsigned w = x.w;
         const double k = kv[it];
         // background of the item's state variable (the FP32 copies are fluctuations)
-        const double bgv = sbg[(w / NN) % NSTATE];
+        const double bgv = sbg[x.s];
         if (st.kout) koutb[w] = k;
         if (st.acc) {
           const double a = (st.acc_init ? ru[it] : ra[it]) + dtb * k;
           accb[w] = a;
-          if (st.ke_part) mb[li[it]] = a;
+          if (st.ke_part) mb[x.li] = a;
           if (st.next32) next32b[w] = (float)(a - bgv);
         }
         if (st.unext) unextb[w] = ru[it] + dta * k;

@@ -275,23 +275,69 @@ __global__ void __launch_bounds__(NT) probe_mfma16(const float *__restrict__ F, 
 
 }  // namespace ldg_probe
 
+namespace {
+
+int launch(int variant, int reps, const float *F, float *O, long K, const ldg_probe::ProbeOps *op, hipStream_t s) {
+  using namespace ldg_probe;
+  const dim3 grid((unsigned)(K / E)), block(NT);
+  if (variant == 0) hipLaunchKernelGGL(probe_valu, grid, block, 0, s, F, O, op, reps);
+  else if (variant == 1) hipLaunchKernelGGL(probe_mfma32, grid, block, 0, s, F, O, op, reps);
+  else hipLaunchKernelGGL(probe_mfma16, grid, block, 0, s, F, O, op, reps);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+}  // namespace
+
 extern "C" {
 
 // Size of the operator block the caller fills (struct ProbeOps).
 int ldg_probe_ops_bytes(void) { return (int)sizeof(ldg_probe::ProbeOps); }
 
-// O = contraction of F (K elements of 15 x 125 floats; K a multiple of 4),
-// variant 0/1/2, `reps` repetitions per workgroup, on `stream`.
-int ldg_probe_contract(int variant, int reps, const float *F, float *O, long K, const void *ops, void *stream) {
+// Host in, host out: copies F (K elements of 15 x 125 floats; K a multiple
+// of 4) and the operator block to the device, launches variant 0/1/2 with
+// `reps` repetitions per workgroup once untimed and `launches` times timed
+// (HIP events on a stream of its own), returns the shortest launch in
+// *best_ms and the last launch's output in O (K x 10 x 125 floats).
+// Allocates and frees its own device memory.  0 on success, -1 for bad
+// arguments, -2 for a HIP error.
+int ldg_probe_measure(int variant, int reps, int launches, const float *F, float *O, long K, const void *ops,
+                      float *best_ms) {
   using namespace ldg_probe;
-  if (K <= 0 || K % E != 0 || reps < 1 || variant < 0 || variant > 2) return -1;
-  const dim3 grid((unsigned)(K / E)), block(NT);
-  hipStream_t s = static_cast<hipStream_t>(stream);
-  const ProbeOps *op = static_cast<const ProbeOps *>(ops);
-  if (variant == 0) hipLaunchKernelGGL(probe_valu, grid, block, 0, s, F, O, op, reps);
-  else if (variant == 1) hipLaunchKernelGGL(probe_mfma32, grid, block, 0, s, F, O, op, reps);
-  else hipLaunchKernelGGL(probe_mfma16, grid, block, 0, s, F, O, op, reps);
-  return hipGetLastError() == hipSuccess ? 0 : -2;
+  if (K <= 0 || K % E != 0 || reps < 1 || launches < 1 || variant < 0 || variant > 2 || !F || !O || !ops || !best_ms)
+    return -1;
+  const size_t fb = (size_t)K * 15 * NN * sizeof(float), ob = (size_t)K * 10 * NN * sizeof(float);
+  float *dF = nullptr, *dO = nullptr;
+  ProbeOps *dops = nullptr;
+  hipStream_t s = nullptr;
+  hipEvent_t a = nullptr, b = nullptr;
+  int rc = -2;
+  if (hipMalloc(&dF, fb) != hipSuccess || hipMalloc(&dO, ob) != hipSuccess ||
+      hipMalloc(&dops, sizeof(ProbeOps)) != hipSuccess || hipStreamCreate(&s) != hipSuccess ||
+      hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess)
+    goto done;
+  if (hipMemcpy(dF, F, fb, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(dops, ops, sizeof(ProbeOps), hipMemcpyHostToDevice) != hipSuccess)
+    goto done;
+  if (launch(variant, reps, dF, dO, K, dops, s) != 0) goto done;
+  *best_ms = -1.0f;
+  for (int i = 0; i < launches; ++i) {
+    float ms = 0.0f;
+    if (hipEventRecord(a, s) != hipSuccess || launch(variant, reps, dF, dO, K, dops, s) != 0 ||
+        hipEventRecord(b, s) != hipSuccess || hipEventSynchronize(b) != hipSuccess ||
+        hipEventElapsedTime(&ms, a, b) != hipSuccess)
+      goto done;
+    if (*best_ms < 0.0f || ms < *best_ms) *best_ms = ms;
+  }
+  if (hipMemcpy(O, dO, ob, hipMemcpyDeviceToHost) != hipSuccess) goto done;
+  rc = 0;
+done:
+  if (a) hipEventDestroy(a);
+  if (b) hipEventDestroy(b);
+  if (s) hipStreamDestroy(s);
+  if (dF) hipFree(dF);
+  if (dO) hipFree(dO);
+  if (dops) hipFree(dops);
+  return rc;
 }
 
 }  // extern "C"

@@ -92,54 +92,39 @@ def reference(F, Ax, Ag, elems):
     return out
 
 
-def measure(elements=64 ** 3, reps=16, samples=64, log=None):
+def measure(elements=64 ** 3, reps=16, samples=64, launches=5, log=None):
     """Times the three variants on `elements` random elements (a multiple of
     4) and checks each against the FP64 contraction on `samples` of them."""
-    import torch
     lib = ctypes.CDLL(os.path.join(HERE, "libldg_probe.so"))
-    lib.ldg_probe_contract.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_long,
-                                       ctypes.c_void_p, ctypes.c_void_p]
+    fp = ctypes.POINTER(ctypes.c_float)
+    lib.ldg_probe_measure.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, fp, fp, ctypes.c_long,
+                                      ctypes.c_void_p, fp]
     It, Dt, Ax, Ag = operators()
     blob = pack(It, Dt, Ax, Ag)
     if len(blob) != lib.ldg_probe_ops_bytes():
         raise RuntimeError(f"operator block {len(blob)} bytes, the library expects {lib.ldg_probe_ops_bytes()}")
     K = elements
     rng = np.random.default_rng(0)
-    F = rng.standard_normal((K, 15, 125)).astype(np.float32)
-    dF = torch.from_numpy(F).cuda()
-    dO = torch.empty((K, 10, 125), dtype=torch.float32, device="cuda")
-    dops = torch.from_numpy(np.frombuffer(blob, dtype=np.uint8).copy()).cuda()
-    stream = torch.cuda.current_stream()
+    F = np.ascontiguousarray(rng.standard_normal((K, 15, 125)).astype(np.float32))
+    O = np.empty((K, 10, 125), dtype=np.float32)
+    ops = ctypes.create_string_buffer(blob, len(blob))
     elems = rng.choice(K, size=min(samples, K), replace=False)
     ref = reference(F, Ax, Ag, elems)
 
-    def launch(v, r):
-        rc = lib.ldg_probe_contract(v, r, dF.data_ptr(), dO.data_ptr(), K, dops.data_ptr(),
-                                    ctypes.c_void_p(stream.cuda_stream))
+    def timed(v, r):
+        best = ctypes.c_float()
+        rc = lib.ldg_probe_measure(v, r, launches, F.ctypes.data_as(fp), O.ctypes.data_as(fp), K, ops,
+                                   ctypes.byref(best))
         if rc != 0:
-            raise RuntimeError(f"ldg_probe_contract returned {rc}")
-
-    def timed(v, r, n=5):
-        launch(v, r)
-        best = None
-        for _ in range(n):
-            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            a.record(stream)
-            launch(v, r)
-            b.record(stream)
-            b.synchronize()
-            t = a.elapsed_time(b)
-            best = t if best is None else min(best, t)
-        return best
+            raise RuntimeError(f"ldg_probe_measure returned {rc}")
+        return best.value
 
     out = {"probe": "y/z test-function contraction of 15 flux fields, p = 4", "elements": K,
            "reps": reps, "variants": {}}
     names = {0: "valu_sum_factorised", 1: "mfma_f32_32x32x2_slab", 2: "mfma_f16_hilo_32x32x16_slab"}
     for v in (0, 1, 2):
-        t1 = timed(v, 1)
-        launch(v, 1)
-        torch.cuda.synchronize()
-        o = dO.cpu().numpy()[elems].astype(np.float64)
+        t1 = timed(v, 1)                       # O now holds one contraction
+        o = O[elems].astype(np.float64)
         err = float(np.linalg.norm(o - ref) / np.linalg.norm(ref))
         tr = timed(v, reps)
         per = (tr - t1) / (reps - 1)
