@@ -62,6 +62,7 @@ struct Group {
   void *ff = nullptr;     // face fluxes written by the gradient kernel, lifted by the rhs kernel
   double *ke_part = nullptr, *diag_part = nullptr, *d_scale = nullptr;
   void *ops = nullptr;
+  void *sops = nullptr;   // the rhs kernel's operator images, one per member
   Tiling rhs[kMaxCand], grad[kMaxCand];
   int crhs = 0, cgrad = 0;   // chosen candidates
   Basis1D basis;

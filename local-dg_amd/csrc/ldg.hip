@@ -180,17 +180,17 @@ void launch_grad_e(ldg_handle *h, Group &g, const Tiling &t, StageIn U, bool wq)
 template <int N, int E>
 void launch_rhs_e(ldg_handle *h, Group &g, const Tiling &t, StageIn U, const StageArgs &st) {
   LaunchArgs la{t.d_inst, t.d_blk, t.nblocks, (t.nblocks + 7) / 8};
-  const Ops<N, double> *op = (const Ops<N, double> *)g.ops;
+  const unsigned char *img = (const unsigned char *)g.sops;
   const dim3 grid(8 * la.per_xcd), block(NT);
   if (h->mixed && U.fluct)
     hipLaunchKernelGGL((ldg_rhs_kernel<N, E, float, float>), grid, block, 0, h->stream, (const float *)U.p,
-                       (const float *)g.ff, la, op, st);
+                       (const float *)g.ff, la, img, st);
   else if (h->mixed)
     hipLaunchKernelGGL((ldg_rhs_kernel<N, E, float, double>), grid, block, 0, h->stream, (const double *)U.p,
-                       (const float *)g.ff, la, op, st);
+                       (const float *)g.ff, la, img, st);
   else
     hipLaunchKernelGGL((ldg_rhs_kernel<N, E, double, double>), grid, block, 0, h->stream, (const double *)U.p,
-                       (const double *)g.ff, la, op, st);
+                       (const double *)g.ff, la, img, st);
 }
 
 template <int N>
@@ -247,6 +247,17 @@ int upload_ops_t(ldg_handle *h, Group &g) {
   if (!(asym < 1e-9)) return fail(h, LDG_EINVAL, "1D operators are not centro-symmetric (node sets must be symmetric)");
   HIPCHECK(h, hipMalloc(&g.ops, sizeof(o)));
   HIPCHECK(h, hipMemcpy(g.ops, &o, sizeof(o), hipMemcpyHostToDevice));
+  // the rhs kernel's operator images, one per member (slot order)
+  std::vector<unsigned char> img((size_t)SOpsImage<N>::BYTES * g.members.size(), 0);
+  for (size_t j = 0; j < g.members.size(); ++j) {
+    const Inst &in = h->inst[g.members[j]];
+    const double a[3] = {2.0 / in.dx[0], 2.0 / in.dx[1], 2.0 / in.dx[2]};
+    unsigned char *b = img.data() + j * SOpsImage<N>::BYTES;
+    fill_sops<N, float>(*reinterpret_cast<SOps<N, float> *>(b + SOpsImage<N>::template offset<float>()), o, a);
+    fill_sops<N, double>(*reinterpret_cast<SOps<N, double> *>(b + SOpsImage<N>::template offset<double>()), o, a);
+  }
+  HIPCHECK(h, hipMalloc(&g.sops, img.size()));
+  HIPCHECK(h, hipMemcpy(g.sops, img.data(), img.size(), hipMemcpyHostToDevice));
   return LDG_OK;
 }
 
@@ -756,6 +767,7 @@ void ldg_destroy(ldg_handle *h) {
     if (g.q) (void)hipFree(g.q);
     if (g.ff) (void)hipFree(g.ff);
     if (g.ops) (void)hipFree(g.ops);
+    if (g.sops) (void)hipFree(g.sops);
     for (int c = 0; c < kMaxCand; ++c) { free_tiling(g.rhs[c]); free_tiling(g.grad[c]); }
     closure_release(g);
   }
@@ -1090,7 +1102,12 @@ static __device__ __align__(256) unsigned char scratch[LDG_NOALLOC_SCRATCH_BYTES
 namespace ldg_host {
 
 constexpr size_t kScrOps = 0, kScrInstGrad = 4096, kScrInstRhs = 6144, kScrFF = 8192;
-static_assert(sizeof(Ops<6, double>) <= kScrInstGrad - kScrOps, "operators do not fit their scratch slot");
+// the rhs kernel's operator image of the precision in use follows the
+// operators in their slot
+template <int N>
+constexpr size_t kScrImg = kScrOps + (sizeof(Ops<N, double>) + 15) / 16 * 16;
+static_assert(kScrImg<6> + SOpsImage<6>::D <= kScrInstGrad && SOpsImage<6>::F <= SOpsImage<6>::D,
+              "operators do not fit their scratch slot");
 static_assert(sizeof(InstDesc) <= kScrInstRhs - kScrInstGrad, "instance table does not fit its scratch slot");
 
 constexpr int kMaxDevices = 64;
@@ -1154,12 +1171,22 @@ template <> const Ops<6, double> &ops_of<6>(const OpsTables &t) { return t.o6; }
 // synchronised (the caller synchronises before returning).
 template <int N>
 hipError_t noalloc_launch(const ldg_config &cfg, const Inst &in, unsigned char *scr, const double *u, double *k,
-                          hipStream_t s, InstDesc dsc[2]) {
+                          hipStream_t s, InstDesc dsc[2], unsigned char *img) {
   constexpr int EG = Cand<N>::grad[0], ER = Cand<N>::rhs[0];
   const Ops<N, double> &o = ops_of<N>(ops_tables());
   if (inst_desc(cfg, in, EG, 0, 0, dsc[0]) != LDG_OK || inst_desc(cfg, in, ER, 0, 0, dsc[1]) != LDG_OK)
     return hipErrorInvalidValue;
+  // the image of the precision in use only, at kScrImg; the kernel is given
+  // the base from which that image's offset lands there
+  const double a[3] = {dsc[1].ialpha[0], dsc[1].ialpha[1], dsc[1].ialpha[2]};
+  const bool mixed = cfg.precision == LDG_MIXED;
+  if (mixed) fill_sops<N, float>(*reinterpret_cast<SOps<N, float> *>(img), o, a);
+  else fill_sops<N, double>(*reinterpret_cast<SOps<N, double> *>(img), o, a);
+  const size_t ib = mixed ? sizeof(SOps<N, float>) : sizeof(SOps<N, double>);
+  const unsigned char *simg =
+      scr + kScrImg<N> - (mixed ? SOpsImage<N>::template offset<float>() : SOpsImage<N>::template offset<double>());
   hipError_t e = hipMemcpyAsync(scr + kScrOps, &o, sizeof(o), hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(scr + kScrImg<N>, img, ib, hipMemcpyHostToDevice, s);
   if (e == hipSuccess) e = hipMemcpyAsync(scr + kScrInstGrad, &dsc[0], sizeof(InstDesc), hipMemcpyHostToDevice, s);
   if (e == hipSuccess) e = hipMemcpyAsync(scr + kScrInstRhs, &dsc[1], sizeof(InstDesc), hipMemcpyHostToDevice, s);
   if (e != hipSuccess) return e;
@@ -1177,13 +1204,13 @@ hipError_t noalloc_launch(const ldg_config &cfg, const Inst &in, unsigned char *
     hipLaunchKernelGGL((ldg_grad_kernel<N, EG, float, double, false>), dim3(8 * lg.per_xcd), dim3(NT), 0, s, u,
                        nullptr, ff, lg, op);
     hipLaunchKernelGGL((ldg_rhs_kernel<N, ER, float, double>), dim3(8 * lr.per_xcd), dim3(NT), 0, s, u,
-                       (const float *)ff, lr, op, st);
+                       (const float *)ff, lr, simg, st);
   } else {
     double *ff = reinterpret_cast<double *>(scr + kScrFF);
     hipLaunchKernelGGL((ldg_grad_kernel<N, EG, double, double, false>), dim3(8 * lg.per_xcd), dim3(NT), 0, s, u,
                        nullptr, ff, lg, op);
     hipLaunchKernelGGL((ldg_rhs_kernel<N, ER, double, double>), dim3(8 * lr.per_xcd), dim3(NT), 0, s, u,
-                       (const double *)ff, lr, op, st);
+                       (const double *)ff, lr, simg, st);
   }
   return hipGetLastError();
 }
@@ -1230,13 +1257,14 @@ extern "C" int ldg_rhs_noalloc(const ldg_config *cfg, const double *u, double *k
   if (!scr) return LDG_EDEVICE;
   hipStream_t s = static_cast<hipStream_t>(stream);
   InstDesc dsc[2];
+  alignas(16) unsigned char img[SOpsImage<6>::D];   // (read by the copies until the synchronisation below)
   hipError_t e = hipErrorInvalidValue;
   switch (n) {
-    case 2: e = noalloc_launch<2>(*cfg, in, scr, u, k, s, dsc); break;
-    case 3: e = noalloc_launch<3>(*cfg, in, scr, u, k, s, dsc); break;
-    case 4: e = noalloc_launch<4>(*cfg, in, scr, u, k, s, dsc); break;
-    case 5: e = noalloc_launch<5>(*cfg, in, scr, u, k, s, dsc); break;
-    case 6: e = noalloc_launch<6>(*cfg, in, scr, u, k, s, dsc); break;
+    case 2: e = noalloc_launch<2>(*cfg, in, scr, u, k, s, dsc, img); break;
+    case 3: e = noalloc_launch<3>(*cfg, in, scr, u, k, s, dsc, img); break;
+    case 4: e = noalloc_launch<4>(*cfg, in, scr, u, k, s, dsc, img); break;
+    case 5: e = noalloc_launch<5>(*cfg, in, scr, u, k, s, dsc, img); break;
+    case 6: e = noalloc_launch<6>(*cfg, in, scr, u, k, s, dsc, img); break;
   }
   const hipError_t es = hipStreamSynchronize(s);
   if (e == hipSuccess) e = es;

@@ -849,35 +849,41 @@ struct SOps {
   double wd[N];
 };
 
-// The operators into LDS, one coefficient per thread (spread over the
-// waves): I, I^T, D^T and D / alpha_l (l = x, y, z) in type T, each
-// coefficient rounded once; the FP64 inverse mass; I L0 / alpha_l; the
-// Gauss weights.  a0..a2 = 1/alpha_l.
+// The rhs kernel's operators for one instance, built on the host from the
+// FP64 operators and the instance's a_l = 1/alpha_l = 2/dx_l: I, I^T, D^T
+// and D a_l (l = x, y, z) in type T, each coefficient rounded once; the
+// FP64 inverse mass; (I L0) a_l; the Gauss weights.  The kernel copies the
+// image of its instance into LDS (no conversion or scaling on the device).
 template <int N, typename T>
-__device__ inline void load_sops(SOps<N, T> *so, const Ops<N, double> *op, double a0, double a1, double a2, int tid) {
+void fill_sops(SOps<N, T> &so, const Ops<N, double> &op, const double a[3]) {
   constexpr int MS = sizeof(SymMat<N, T>) / sizeof(T);   // coefficients per operator
   static_assert(sizeof(SymMat<N, T>) == MS * sizeof(T) && sizeof(SymMat<N, double>) == MS * sizeof(double),
                 "operators are plain coefficient arrays");
   using SO = SOps<N, T>;
   static_assert(offsetof(SO, Ds) == 3 * sizeof(SymMat<N, T>), "I, I^T, D^T, D_x, D_y, D_z are contiguous");
-  constexpr int B1 = 6 * MS, B2 = B1 + MS, B3 = B2 + 3 * N, B4 = B3 + N;
-  static_assert(B4 + N <= NT, "one coefficient per thread");
-  if (tid < B1) {
-    const int o = tid / MS, i = tid - o * MS;
-    const SymMat<N, double> *src = o == 0 ? &op->sI : o == 1 ? &op->sIt : o == 2 ? &op->sDt : &op->sD;
-    const double f = o < 3 ? 1.0 : o == 3 ? a0 : o == 4 ? a1 : a2;
-    reinterpret_cast<T *>(&so->I)[tid] = T(reinterpret_cast<const double *>(src)[i] * f);
-  } else if (tid < B2) {
-    reinterpret_cast<double *>(&so->Mi)[tid - B1] = reinterpret_cast<const double *>(&op->sMi)[tid - B1];
-  } else if (tid < B3) {
-    const int l = (tid - B2) / N, q = tid - B2 - l * N;
-    so->il0[l][q] = T(op->IL0[q] * (l == 0 ? a0 : l == 1 ? a1 : a2));
-  } else if (tid < B4) {
-    so->w[tid - B3] = T(op->w[tid - B3]);
-  } else if (tid < B4 + N) {
-    so->wd[tid - B4] = op->w[tid - B4];
+  const SymMat<N, double> *src[6] = {&op.sI, &op.sIt, &op.sDt, &op.sD, &op.sD, &op.sD};
+  const double f[6] = {1.0, 1.0, 1.0, a[0], a[1], a[2]};
+  T *dst = reinterpret_cast<T *>(&so.I);
+  for (int o = 0; o < 6; ++o)
+    for (int i = 0; i < MS; ++i) dst[o * MS + i] = T(reinterpret_cast<const double *>(src[o])[i] * f[o]);
+  so.Mi = op.sMi;
+  for (int l = 0; l < 3; ++l)
+    for (int q = 0; q < N; ++q) so.il0[l][q] = T(op.IL0[q] * a[l]);
+  for (int q = 0; q < N; ++q) {
+    so.w[q] = T(op.w[q]);
+    so.wd[q] = op.w[q];
   }
 }
+
+// Per instance, the FP32 image then the FP64 image, each padded to 16 bytes.
+template <int N>
+struct SOpsImage {
+  static constexpr int F = (int)((sizeof(SOps<N, float>) + 15) / 16 * 16);
+  static constexpr int D = (int)((sizeof(SOps<N, double>) + 15) / 16 * 16);
+  static constexpr int BYTES = F + D;
+  template <typename T>
+  static constexpr int offset() { return sizeof(T) == sizeof(double) ? F : 0; }
+};
 
 template <int N, typename T>
 struct Mat {
@@ -892,7 +898,7 @@ struct Mat {
 
 template <int N, int E, typename T, typename UT>
 __global__ void __launch_bounds__(NT, (RhsLds<N, T>::WAVES))
-ldg_rhs_kernel(const UT *__restrict__ U, const T *__restrict__ FF, LaunchArgs la, const Ops<N, double> *__restrict__ opp,
+ldg_rhs_kernel(const UT *__restrict__ U, const T *__restrict__ FF, LaunchArgs la, const unsigned char *__restrict__ simg,
                StageArgs st) {
   using L = RhsLds<N, T>;
   constexpr int NN = L::NN, NF = L::NF, BUF = L::BUF;
@@ -1001,7 +1007,15 @@ ldg_rhs_kernel(const UT *__restrict__ U, const T *__restrict__ FF, LaunchArgs la
   }
 
   fill_neighbors(m, ec, d.elem_off, ne, snb, tid);
-  load_sops<N, T>(&sops, opp, m.ialpha[0], m.ialpha[1], m.ialpha[2], tid);
+  {
+    // the instance's operator image (8-byte words) into LDS
+    static_assert(sizeof(SOps<N, T>) % 8 == 0, "whole 8-byte words");
+    constexpr int W8 = sizeof(SOps<N, T>) / 8;
+    const int ii = la.blk_inst ? la.blk_inst[lb] : 0;
+    const uint2 *src = reinterpret_cast<const uint2 *>(simg + (long)ii * SOpsImage<N>::BYTES +
+                                                       SOpsImage<N>::template offset<T>());
+    for (int w = tid; w < W8; w += NT) reinterpret_cast<uint2 *>(&sops)[w] = src[w];
+  }
   if (tid < NSTATE) sbg[tid] = tid == 0 ? ph.rho0 : tid == NSTATE - 1 ? ph.E0 : 0.0;
   __syncthreads();
   // the neighbours' traces the jumps need: the -x neighbour's node at the
