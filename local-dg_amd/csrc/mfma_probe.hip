@@ -331,12 +331,12 @@ int ldg_probe_measure(int variant, int reps, int launches, const float *F, float
   if (hipMemcpy(O, dO, ob, hipMemcpyDeviceToHost) != hipSuccess) goto done;
   rc = 0;
 done:
-  if (a) hipEventDestroy(a);
-  if (b) hipEventDestroy(b);
-  if (s) hipStreamDestroy(s);
-  if (dF) hipFree(dF);
-  if (dO) hipFree(dO);
-  if (dops) hipFree(dops);
+  if (a) (void)hipEventDestroy(a);
+  if (b) (void)hipEventDestroy(b);
+  if (s) (void)hipStreamDestroy(s);
+  if (dF) (void)hipFree(dF);
+  if (dO) (void)hipFree(dO);
+  if (dops) (void)hipFree(dops);
   return rc;
 }
 
