@@ -256,10 +256,16 @@ int ldg_closure_train(ldg_closure *c, int epochs, int batch, double lr, double v
    element, unless the RMS of the normalised correction exceeds `bound`
    (that element keeps the plain right-hand side; bound 0 switches every
    element off) or, with flags & LDG_CLOSURE_DISSIPATIVE, unless the
-   correction would raise the element's nodal kinetic energy.  c must
-   outlive the attachment.  ldg_closure_gated: elements switched off in the
-   last evaluated stage, of `total`. */
+   correction would raise the element's nodal kinetic energy, or, with
+   flags & LDG_CLOSURE_ENERGIZING, unless it would lower it (not both).
+   With flags & LDG_CLOSURE_PER_STEP, ldg_step evaluates the correction
+   and the gate once per RK4 step, from the step's initial state in stage
+   1, and adds them in all four stages (ldg_rhs always evaluates them).  c
+   must outlive the attachment.  ldg_closure_gated: elements switched off in
+   the last evaluated stage, of `total`. */
 #define LDG_CLOSURE_DISSIPATIVE 1
+#define LDG_CLOSURE_ENERGIZING 2
+#define LDG_CLOSURE_PER_STEP 4
 int ldg_set_closure(ldg_handle *h, const ldg_closure *c, double bound, int flags);
 int ldg_closure_gated(ldg_handle *h, long *off, long *total);
 

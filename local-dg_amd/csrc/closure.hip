@@ -288,9 +288,14 @@ void free_closure(ldg_closure *c) {
 
 namespace ldg_host {
 
-int closure_stage(ldg_handle *h, Group &g, StageIn U, StageArgs &st) {
+int closure_stage(ldg_handle *h, Group &g, StageIn U, StageArgs &st, int stage) {
   GroupClosure &cl = g.cl;
   if (!cl.net) return LDG_OK;
+  if (cl.per_step && stage > 0) {   // stage 1's correction and gate, held for the step
+    st.corr = cl.corr;
+    st.gate = cl.gate;
+    return LDG_OK;
+  }
   const ldg_closure *c = cl.net;
   const int dofs = g.n * g.n * g.n;
   if (fused_ok(c)) {
@@ -772,7 +777,9 @@ int ldg_closure_train(ldg_closure *c, int epochs, int batch, double lr, double v
 }
 
 int ldg_set_closure(ldg_handle *h, const ldg_closure *c, double bound, int flags) {
-  if (!h || (c && !(bound >= 0)) || (flags & ~LDG_CLOSURE_DISSIPATIVE)) return LDG_EINVAL;
+  const int both = LDG_CLOSURE_DISSIPATIVE | LDG_CLOSURE_ENERGIZING;
+  if (!h || (c && !(bound >= 0)) || (flags & ~(both | LDG_CLOSURE_PER_STEP)) || (flags & both) == both)
+    return LDG_EINVAL;
   HIPCHECK(h, hipStreamSynchronize(h->stream));
   drop_graph(h);
   for (Group &g : h->groups) {
@@ -791,7 +798,10 @@ int ldg_set_closure(ldg_handle *h, const ldg_closure *c, double bound, int flags
     }
     cl.net = c;
     cl.bound = bound;
-    cl.dissipative = (flags & LDG_CLOSURE_DISSIPATIVE) ? 1 : 0;
+    // the energy gate's sign: +1 keeps corrections that do not raise the
+    // element's kinetic energy, -1 those that do not lower it, 0 none
+    cl.dissipative = (flags & LDG_CLOSURE_DISSIPATIVE) ? 1 : (flags & LDG_CLOSURE_ENERGIZING) ? -1 : 0;
+    cl.per_step = (flags & LDG_CLOSURE_PER_STEP) != 0;
   }
   return LDG_OK;
 }

@@ -167,8 +167,9 @@ __device__ inline double ke_production(const UT *__restrict__ u, double rho_add,
 
 // Closure correction of each element (one 64-thread block per element):
 // corr = Y * ostd + omean, applied (gate = 1) only if the RMS of the
-// normalised output Y is at most `bound` and, with `dissipative`, if it
-// does not raise the element's kinetic energy; else gate = 0 (plain RHS).
+// normalised output Y is at most `bound` and, with `dissipative` = 1, if it
+// does not raise the element's kinetic energy (= -1: does not lower it);
+// else gate = 0 (plain RHS).
 template <typename UT>
 __global__ void __launch_bounds__(64) mlp_correction(const float *__restrict__ Y, long K, int nout,
                                                      const float *__restrict__ omean, const float *__restrict__ ostd,
@@ -182,7 +183,8 @@ __global__ void __launch_bounds__(64) mlp_correction(const float *__restrict__ Y
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) ss += __shfl_xor(ss, o, 64);
     bool on = sqrt((double)ss / nout) <= bound;
-    if (on && dissipative) on = ke_production(U + e * nout, rho_add, nout / 5, y, ostd, omean, threadIdx.x) <= 0.0;
+    if (on && dissipative)
+      on = dissipative * ke_production(U + e * nout, rho_add, nout / 5, y, ostd, omean, threadIdx.x) <= 0.0;
     for (int j = threadIdx.x; j < nout; j += 64)
       corr[e * nout + j] = on ? (double)y[j] * (double)ostd[j] + (double)omean[j] : 0.0;
     if (threadIdx.x == 0) gate[e] = on ? 1 : 0;
@@ -479,7 +481,7 @@ __global__ void __launch_bounds__(256) mlp_infer(const UT *__restrict__ U, const
     bool on = sqrt((double)ss / NOUT) <= bound;
     const long e = e0 + r;
     if (on && dissipative)
-      on = ke_production(U + e * NU, rho_add, DOFS, &Ys[r][0], ostd, omean, lane) <= 0.0;
+      on = dissipative * ke_production(U + e * NU, rho_add, DOFS, &Ys[r][0], ostd, omean, lane) <= 0.0;
     for (int j = lane; j < NOUT; j += 64)
       corr[e * NOUT + j] = on ? (double)Ys[r][j] * (double)ostd[j] + (double)omean[j] : 0.0;
     if (lane == 0) gate[e] = on ? 1 : 0;

@@ -44,7 +44,8 @@ struct Tiling {
 struct GroupClosure {
   const ldg_closure *net = nullptr;
   double bound = 0.0;                 // gate: correction RMS in units of the training spread
-  int dissipative = 0;                // gate: also switch off corrections that raise the element's KE
+  int dissipative = 0;                // energy gate: +1 off when a correction raises the element's KE, -1 when it lowers it
+  bool per_step = false;              // correction evaluated in stage 1 of each RK4 step and applied in all four
   float *X = nullptr, *H1 = nullptr, *H2 = nullptr, *Y = nullptr;
   double *corr = nullptr;             // per element, state layout
   unsigned char *gate = nullptr;      // per element: 1 = correction applied
@@ -135,7 +136,7 @@ size_t inst_values(const ldg_handle *h, int i);
 // closure.hip: before a stage's rhs launch, evaluates the group's closure on
 // the stage input U (and the gradient just computed into g.q) and points st
 // at the correction; no-op without a closure.
-int closure_stage(ldg_handle *h, Group &g, StageIn U, StageArgs &st);
+int closure_stage(ldg_handle *h, Group &g, StageIn U, StageArgs &st, int stage = 0);
 void closure_release(Group &g);
 
 }  // namespace ldg_host

@@ -511,7 +511,9 @@ int rk4_step(ldg_handle *h, bool ke, bool in32, bool out32) {
     StageIn U = carry && in32 ? StageIn{g.ub, true} : state_in(g.u);
     void *bufs[2] = {g.ua, g.ub};
     for (int s = 0; s < 4; ++s) {
-      int rc = launch_grad(h, g, U);
+      // (a closure evaluated once per step reads the nodal gradient in
+      // stage 1 only)
+      int rc = launch_grad(h, g, U, -1, g.cl.net && g.cl.per_step && s > 0 ? 0 : -1);
       if (rc) return rc;
       StageArgs st = empty_stage();
       st.u0 = g.u;
@@ -525,7 +527,7 @@ int rk4_step(ldg_handle *h, bool ke, bool in32, bool out32) {
       }
       if (s == 3 && ke) st.ke_part = g.ke_part;
       if (s == 3 && carry && out32) st.next32 = (float *)g.ub;
-      rc = closure_stage(h, g, U, st);
+      rc = closure_stage(h, g, U, st, s);
       if (rc) return rc;
       rc = launch_rhs(h, g, U, st);
       if (rc) return rc;

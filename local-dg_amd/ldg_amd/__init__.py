@@ -26,6 +26,8 @@ LIB_PATH = os.path.join(_HERE, "libldg.so")
 LDG_OK, LDG_EINVAL, LDG_ENOMEM, LDG_EDEVICE, LDG_ENONFINITE = 0, -1, -2, -3, -4
 LDG_FP64, LDG_MIXED = 0, 1
 LDG_CLOSURE_DISSIPATIVE = 1
+LDG_CLOSURE_ENERGIZING = 2
+LDG_CLOSURE_PER_STEP = 4
 
 EXPORTED = [
     "ldg_config_default", "ldg_create", "ldg_destroy", "ldg_last_error", "ldg_sizes",

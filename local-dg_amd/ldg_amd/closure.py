@@ -24,7 +24,8 @@ import time
 
 import numpy as np
 
-from . import LDG, LDG_CLOSURE_DISSIPATIVE, LDG_FP64, LDG_MIXED, LDG_OK, Ensemble, LDGError, load_library
+from . import (LDG, LDG_CLOSURE_DISSIPATIVE, LDG_CLOSURE_ENERGIZING, LDG_CLOSURE_PER_STEP, LDG_FP64, LDG_MIXED, LDG_OK,
+               Ensemble, LDGError, load_library)
 
 _fp = ctypes.POINTER(ctypes.c_float)
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -167,12 +168,16 @@ def project_state(fine, i, coarse, j):
         raise LDGError(rc, lib.ldg_last_error(coarse._h).decode())
 
 
-def attach(solver, closure, bound, dissipative=False):
+def attach(solver, closure, bound, dissipative=False, per_step=False):
     """ldg_set_closure: closure=None detaches; dissipative=True also gates
-    off corrections that would raise an element's kinetic energy."""
+    off corrections that would raise an element's kinetic energy,
+    dissipative="energizing" those that would lower it; per_step=True
+    evaluates the correction once per RK4 step (stage 1) and adds it in
+    all four stages."""
     lib = load_library()
     rc = lib.ldg_set_closure(solver._h, closure._c if closure is not None else None, bound,
-                             LDG_CLOSURE_DISSIPATIVE if dissipative else 0)
+                             (LDG_CLOSURE_ENERGIZING if dissipative == "energizing" else
+                              LDG_CLOSURE_DISSIPATIVE if dissipative else 0) | (LDG_CLOSURE_PER_STEP if per_step else 0))
     if rc != LDG_OK:
         raise LDGError(rc, lib.ldg_last_error(solver._h).decode())
 
@@ -244,7 +249,7 @@ def generate(closure, runs=4, p_fine=5, ne=24, t_end=10.0, dt_snap=0.1, keep=0.2
 
 
 def run_tgv(p, ne, t_end, closure=None, bound=5.0, precision=LDG_FP64, cfl=0.01, dt_diag=0.1, Re=1600.0,
-            M0=0.1, log=None, dissipative=False, ic_from=None, t_on=0.0):
+            M0=0.1, log=None, dissipative=False, ic_from=None, t_on=0.0, per_step=False):
     """Taylor-Green at order p on ne^3 to t_end (reference time-step rule);
     returns times and reference KE after every step, Gauss-point KE and
     enstrophy every dt_diag, wall time of the stepping, gated elements.
@@ -256,7 +261,7 @@ def run_tgv(p, ne, t_end, closure=None, bound=5.0, precision=LDG_FP64, cfl=0.01,
     k_on = int(round(t_on / dt_diag))
     attached = closure is not None and k_on <= 0
     if attached:
-        attach(g, closure, bound, dissipative)
+        attach(g, closure, bound, dissipative, per_step)
     if ic_from:
         f = LDG(ic_from, ne, Re=Re, M0=M0)
         f.initial_condition()
@@ -272,7 +277,7 @@ def run_tgv(p, ne, t_end, closure=None, bound=5.0, precision=LDG_FP64, cfl=0.01,
     diverged = None
     for k in range(ndiag):
         if closure is not None and not attached and k >= k_on:
-            attach(g, closure, bound, dissipative)
+            attach(g, closure, bound, dissipative, per_step)
             attached = True
         t0 = time.perf_counter()
         try:
@@ -396,10 +401,12 @@ def main(argv=None):
     ev.add_argument("--ne", type=int, default=24)
     ev.add_argument("--t-end", type=float, default=10.0)
     ev.add_argument("--bounds", default="5", help="comma-separated gate bounds")
-    ev.add_argument("--gates", default="plain", help="comma-separated: plain, dissipative")
+    ev.add_argument("--gates", default="plain", help="comma-separated: plain, dissipative, energizing")
     ev.add_argument("--relax", default="1", help="comma-separated factors multiplying the correction")
     ev.add_argument("--ic-from", type=int, default=0, help="start p=2 runs from the projected order-q state")
     ev.add_argument("--t-on", default="0", help="comma-separated times from which the closure is attached")
+    ev.add_argument("--per-step", action="store_true", help="evaluate the correction once per RK4 step")
+    ev.add_argument("--precision", choices=["fp64", "mixed"], default="fp64", help="precision of the p=2 runs")
     ev.add_argument("--ref-p", type=int, default=5)
     ev.add_argument("--ref-json", default=None, help="take the resolved run from an earlier evaluate output")
     ev.add_argument("--no-ref", action="store_true")
@@ -449,6 +456,7 @@ def main(argv=None):
         print(json.dumps({k: v for k, v in info.items() if k != "instances"}), flush=True)
     else:
         c = Closure.load(a.weights)
+        prec2 = LDG_MIXED if a.precision == "mixed" else LDG_FP64
         res, runs = {}, {}
         params, norm = c.get()
         nin, nout = c.nin, c.nout
@@ -460,10 +468,11 @@ def main(argv=None):
                 for gate in a.gates.split(","):
                     for t_on in [float(t) for t in a.t_on.split(",")]:
                         tag = f"closure_b{bound:g}_{gate}_r{relax:g}" + (f"_on{t_on:g}" if t_on > 0 else "")
-                        runs[tag] = run_tgv(2, a.ne, a.t_end, c, bound, log=log, dissipative=(gate == "dissipative"),
-                                            ic_from=a.ic_from, t_on=t_on)
+                        runs[tag] = run_tgv(2, a.ne, a.t_end, c, bound, precision=prec2, log=log,
+                                            dissipative=("energizing" if gate == "energizing" else gate == "dissipative"),
+                                            ic_from=a.ic_from, t_on=t_on, per_step=a.per_step)
         c.set(norm=norm)
-        runs["plain_p2"] = run_tgv(2, a.ne, a.t_end, log=log, ic_from=a.ic_from)
+        runs["plain_p2"] = run_tgv(2, a.ne, a.t_end, precision=prec2, log=log, ic_from=a.ic_from)
         ref = None
         if a.ref_json:
             d = json.load(open(a.ref_json))[f"plain_p{a.ref_p}"]

@@ -201,17 +201,19 @@ def test_training_is_deterministic_and_learns_a_teacher():
 
 
 @pytest.mark.parametrize("hidden", [256, 64])
-def test_dissipative_gate_keeps_only_energy_removing_corrections(hidden):
+@pytest.mark.parametrize("mode", ["dissipative", "energizing"])
+def test_energy_gate_keeps_only_corrections_of_one_sign(hidden, mode):
     """With the dissipative gate an element keeps its correction only if
     the correction does not raise its nodal kinetic energy
-    sum_j |m_j|^2 / (2 rho_j); the others keep the plain right-hand side."""
+    sum_j |m_j|^2 / (2 rho_j), with the energizing gate only if it does not
+    lower it; the others keep the plain right-hand side."""
     plain = ldg_amd.LDG(2, 4)
     corr = ldg_amd.LDG(2, 4)
     c = C.Closure(2, hidden, 0, seed=8)
     rng = np.random.default_rng(5)
     norm = random_norm(c, rng)
     c.set(norm=norm)
-    C.attach(corr, c, 1e30, dissipative=True)
+    C.attach(corr, c, 1e30, dissipative=True if mode == "dissipative" else "energizing")
     plain.initial_condition()
     u = plain.get_state()
     k0, du = plain.rhs(u, want_du=True)
@@ -227,7 +229,7 @@ def test_dissipative_gate_keeps_only_energy_removing_corrections(hidden):
     full = (Y * norm[2 * nin + nout:] + norm[2 * nin:2 * nin + nout]).reshape(k0.shape)
     v = u[:, 1:4] / u[:, 0:1]
     prod = np.sum(np.sum(v * full[:, 1:4], axis=1) - 0.5 * np.sum(v * v, axis=1) * full[:, 0], axis=1)
-    on = prod <= 0
+    on = prod <= 0 if mode == "dissipative" else prod >= 0
     off, total = C.gated(corr)
     assert total == K and off == int(np.sum(~on)) and 0 < off < K
     d = k1 - k0
@@ -291,6 +293,40 @@ def test_late_switch_on_matches_plain_until_attached():
     np.testing.assert_array_equal(late["ke_ref"][:n_on], plain["ke_ref"][:n_on])
     np.testing.assert_array_equal(late["ke_gauss"][:3], plain["ke_gauss"][:3])
     assert late["ke_gauss"][3] != plain["ke_gauss"][3]
+
+
+def test_per_step_closure_holds_the_stage_one_correction():
+    """LDG_CLOSURE_PER_STEP: the correction c0 of the step's initial state
+    u0 is added in all four stages.  One step equals the RK4 step (the
+    reference tableau) of k(U) + c0, restated here with the plain
+    right-hand side k of the same library, to FP64 round-off."""
+    c = C.Closure(2, 256, 0, seed=21)
+    c.set(norm=random_norm(c, np.random.default_rng(7)))
+    plain, probe, corr = ldg_amd.LDG(2, 4), ldg_amd.LDG(2, 4), ldg_amd.LDG(2, 4)
+    C.attach(probe, c, 1e30)
+    C.attach(corr, c, 1e30, per_step=True)
+    corr.initial_condition()
+    u0 = corr.get_state()
+    c0 = probe.rhs(u0) - plain.rhs(u0)
+    assert np.max(np.abs(c0)) > 0
+    dt = corr.time_step(0.01)
+    a, b = [0.5, 0.5, 1.0], [1 / 6, 1 / 3, 1 / 3, 1 / 6]
+    U, acc = u0, u0.copy()
+    for st in range(4):
+        k = plain.rhs(U) + c0
+        acc = acc + dt * b[st] * k
+        if st < 3:
+            U = u0 + dt * a[st] * k
+    corr.step(dt, 1, record_ke=False)
+    e = rel(corr.get_state(), acc)
+    print("per-step closure vs restated RK4 step:", e, flush=True)
+    assert e <= 1e-13
+    # the per-stage closure (correction of every stage's input) differs
+    both = ldg_amd.LDG(2, 4)
+    C.attach(both, c, 1e30)
+    both.initial_condition()
+    both.step(dt, 1, record_ke=False)
+    assert not np.array_equal(both.get_state(), corr.get_state())
 
 
 def test_adam_matches_numpy_adam():
