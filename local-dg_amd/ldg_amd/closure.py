@@ -406,6 +406,7 @@ def main(argv=None):
     ev.add_argument("--ic-from", type=int, default=0, help="start p=2 runs from the projected order-q state")
     ev.add_argument("--t-on", default="0", help="comma-separated times from which the closure is attached")
     ev.add_argument("--per-step", action="store_true", help="evaluate the correction once per RK4 step")
+    ev.add_argument("--re", type=float, default=1600.0, help="Reynolds number of the evaluation case")
     ev.add_argument("--precision", choices=["fp64", "mixed"], default="fp64", help="precision of the p=2 runs")
     ev.add_argument("--ref-p", type=int, default=5)
     ev.add_argument("--ref-json", default=None, help="take the resolved run from an earlier evaluate output")
@@ -468,18 +469,18 @@ def main(argv=None):
                 for gate in a.gates.split(","):
                     for t_on in [float(t) for t in a.t_on.split(",")]:
                         tag = f"closure_b{bound:g}_{gate}_r{relax:g}" + (f"_on{t_on:g}" if t_on > 0 else "")
-                        runs[tag] = run_tgv(2, a.ne, a.t_end, c, bound, precision=prec2, log=log,
+                        runs[tag] = run_tgv(2, a.ne, a.t_end, c, bound, precision=prec2, Re=a.re, log=log,
                                             dissipative=("energizing" if gate == "energizing" else gate == "dissipative"),
                                             ic_from=a.ic_from, t_on=t_on, per_step=a.per_step)
         c.set(norm=norm)
-        runs["plain_p2"] = run_tgv(2, a.ne, a.t_end, precision=prec2, log=log, ic_from=a.ic_from)
+        runs["plain_p2"] = run_tgv(2, a.ne, a.t_end, precision=prec2, Re=a.re, log=log, ic_from=a.ic_from)
         ref = None
         if a.ref_json:
             d = json.load(open(a.ref_json))[f"plain_p{a.ref_p}"]
             ref = {"t_diag": np.array(d["t"]), "ke_gauss": np.array(d["ke_gauss"]), "wall_s": d["wall_s"],
                    "steps": d["steps"], "gated_max": 0, "enstrophy": np.array(d["enstrophy"])}
         elif not a.no_ref:
-            ref = run_tgv(a.ref_p, a.ne, a.t_end, log=log)
+            ref = run_tgv(a.ref_p, a.ne, a.t_end, Re=a.re, log=log)
         if ref is not None:
             runs[f"plain_p{a.ref_p}"] = ref
             for k, r in runs.items():
