@@ -329,6 +329,19 @@ def test_per_step_closure_holds_the_stage_one_correction():
     assert not np.array_equal(both.get_state(), corr.get_state())
 
 
+def test_set_closure_refuses_bad_flags():
+    """Both energy gates at once, or an unknown bit, are refused."""
+    g = ldg_amd.LDG(2, 4)
+    c = C.Closure(2, 256, 0, seed=3)
+    lib = ldg_amd.load_library()
+    both = ldg_amd.LDG_CLOSURE_DISSIPATIVE | ldg_amd.LDG_CLOSURE_ENERGIZING
+    assert lib.ldg_set_closure(g._h, c._c, 1.0, both) == ldg_amd.LDG_EINVAL
+    assert lib.ldg_set_closure(g._h, c._c, 1.0, 8) == ldg_amd.LDG_EINVAL
+    assert lib.ldg_set_closure(g._h, c._c, 1.0, ldg_amd.LDG_CLOSURE_ENERGIZING | ldg_amd.LDG_CLOSURE_PER_STEP) == \
+        ldg_amd.LDG_OK
+    assert lib.ldg_set_closure(g._h, None, 0.0, 0) == ldg_amd.LDG_OK
+
+
 def test_adam_matches_numpy_adam():
     """Five full-batch Adam steps of ldg_closure_train against Algorithm 1
     of Kingma & Ba (2015) in float64 numpy: m_t = b1 m + (1-b1) g,
